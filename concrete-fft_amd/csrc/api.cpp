@@ -1,0 +1,675 @@
+// api.cpp -- C ABI of libcfft_amd: plans, f64 transforms, permuted-order
+// helpers and the fused negacyclic product.  Kernels live in *.hip files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "cfft.h"
+#include "host_tables.hpp"
+#include "launch.hpp"
+#include "plan.hpp"
+
+using cfft_launch::dc64;
+using cfft_prog::Pass;
+using cfft_prog::Program;
+
+namespace cfft_internal {
+
+static thread_local std::string g_last_error;
+
+int set_error(int status, const std::string &msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int hip_error(hipError_t e, const char *where) {
+  if (e == hipSuccess) return CFFT_SUCCESS;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return set_error(CFFT_ERROR_OUT_OF_MEMORY, std::string(where) + ": " + hipGetErrorString(e));
+  if (e == hipErrorNoDevice)
+    return set_error(CFFT_ERROR_NO_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+  return set_error(CFFT_ERROR_DEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace cfft_internal
+
+using cfft_internal::hip_error;
+using cfft_internal::set_error;
+
+#define CFFT_HIP(call)                                  \
+  do {                                                  \
+    hipError_t e__ = (call);                            \
+    if (e__ != hipSuccess) return hip_error(e__, #call); \
+  } while (0)
+
+cfft_plan_s::~cfft_plan_s() {
+  if (d_tw) (void)hipFree(d_tw);
+  if (d_freq) (void)hipFree(d_freq);
+  if (d_pos) (void)hipFree(d_pos);
+  if (d_twist) (void)hipFree(d_twist);
+  if (d_untwist) (void)hipFree(d_untwist);
+  if (d_tw128) (void)hipFree(d_tw128);
+}
+
+namespace {
+
+bool is_pow2(size_t n) { return n != 0 && (n & (n - 1)) == 0; }
+uint32_t ilog2(size_t n) {
+  uint32_t l = 0;
+  while (((size_t)1 << l) < n) ++l;
+  return l;
+}
+int algo_radix(int algo) {
+  switch (algo) {
+    case CFFT_DIF2: case CFFT_DIT2: return 2;
+    case CFFT_DIF4: case CFFT_DIT4: return 4;
+    case CFFT_DIF8: case CFFT_DIT8: return 8;
+    case CFFT_DIF16: case CFFT_DIT16: return 16;
+    default: return 0;
+  }
+}
+bool algo_is_dit(int algo) {
+  return algo == CFFT_DIT2 || algo == CFFT_DIT4 || algo == CFFT_DIT8 || algo == CFFT_DIT16;
+}
+const char *algo_name(int algo) {
+  static const char *names[] = {"Dif2", "Dit2", "Dif4", "Dit4", "Dif8", "Dit8", "Dif16", "Dit16"};
+  return (algo >= 0 && algo < 8) ? names[algo] : "?";
+}
+
+Pass mkpass(uint8_t kind, int radix, bool fwd, uint32_t log2len, uint32_t log2s, uint32_t log2tw) {
+  Pass p{};
+  p.kind = kind;
+  p.radix = (uint8_t)radix;
+  p.fwd = fwd ? 1 : 0;
+  p.log2len = (uint8_t)log2len;
+  p.s = log2s;
+  p.tw_mul = log2tw;
+  return p;
+}
+
+// Stockham recursion of src/dif{2,4,8,16}.rs / src/dit{2,4,8,16}.rs as a
+// pass list: core passes while more than log2(R) levels remain, then the end
+// butterfly of radix 2^L; DIT runs the end first and the cores in reverse.
+void base_passes(std::vector<Pass> &out, int algo, uint32_t log2bn, bool fwd, uint32_t log2n) {
+  const int R = algo_radix(algo);
+  const uint32_t lr = ilog2((size_t)R);
+  const uint32_t tw = log2n - log2bn;
+  uint32_t L = log2bn;
+  if (L == 0) return;  // n == 1: do_nothing (src/ordered.rs:204)
+  std::vector<uint32_t> cores;
+  uint32_t s = 0;
+  while (L > lr) {
+    cores.push_back(s);
+    s += lr;
+    L -= lr;
+  }
+  const int rend = 1 << L;
+  if (!algo_is_dit(algo)) {
+    for (uint32_t cs : cores) out.push_back(mkpass(cfft_prog::DIF_CORE, R, fwd, log2bn, cs, tw));
+    out.push_back(mkpass(cfft_prog::END, rend, fwd, log2bn, 0, tw));
+  } else {
+    out.push_back(mkpass(cfft_prog::END, rend, fwd, log2bn, 0, tw));
+    for (auto it = cores.rbegin(); it != cores.rend(); ++it)
+      out.push_back(mkpass(cfft_prog::DIT_CORE, R, fwd, log2bn, *it, tw));
+  }
+}
+
+// depth radix choice, src/unordered.rs:361-367 and :407-413
+int depth_radix(size_t n, size_t base_n) {
+  if (n == 2 * base_n) return 2;
+  if (n == 4 * base_n) return 4;
+  return 8;
+}
+
+void full_program(std::vector<Pass> &fwd, std::vector<Pass> &inv, size_t n, size_t base_n,
+                  int algo) {
+  const uint32_t log2n = ilog2(n), log2bn = ilog2(base_n);
+  std::vector<std::pair<int, uint32_t>> levels;  // (radix, log2 len), top first
+  size_t nl = n;
+  while (nl > base_n) {
+    const int r = depth_radix(nl, base_n);
+    levels.push_back({r, ilog2(nl)});
+    nl /= (size_t)r;
+  }
+  for (auto &lv : levels)
+    fwd.push_back(mkpass(cfft_prog::DEPTH_FWD, lv.first, true, lv.second, 0, log2n - lv.second));
+  base_passes(fwd, algo, log2bn, true, log2n);
+  base_passes(inv, algo, log2bn, false, log2n);
+  for (auto it = levels.rbegin(); it != levels.rend(); ++it)
+    inv.push_back(mkpass(cfft_prog::DEPTH_INV, it->first, false, it->second, 0, log2n - it->second));
+}
+
+bool to_program(const std::vector<Pass> &v, size_t first, size_t last, Program &P) {
+  if (last - first > (size_t)cfft_prog::kMaxPasses) return false;
+  P = Program{};
+  P.npass = (uint32_t)(last - first);
+  uint32_t chunk = 0;
+  for (size_t i = first; i < last; ++i) {
+    P.p[i - first] = v[i];
+    chunk = std::max<uint32_t>(chunk, v[i].log2len);
+  }
+  P.log2n = chunk;
+  return true;
+}
+
+int build_f64_plan(cfft_plan_s *p) {
+  const size_t n = p->n;
+  const uint32_t log2n = ilog2(n);
+  std::vector<Pass> fwd, inv;
+  full_program(fwd, inv, n, p->base_n, p->algo);
+  // whole-transform programs for fused kernels
+  p->fwd_full = Program{};
+  p->inv_full = Program{};
+  if (log2n <= cfft_launch::kLdsMaxLog2) {
+    to_program(fwd, 0, fwd.size(), p->fwd_full);
+    to_program(inv, 0, inv.size(), p->inv_full);
+    p->fwd_full.log2n = log2n;
+    p->inv_full.log2n = log2n;
+  }
+  // split off depth passes larger than an LDS chunk
+  size_t fg = 0;
+  while (fg < fwd.size() && fwd[fg].log2len > cfft_launch::kLdsMaxLog2) ++fg;
+  size_t ig = inv.size();
+  while (ig > 0 && inv[ig - 1].log2len > cfft_launch::kLdsMaxLog2) --ig;
+  p->fwd_global.assign(fwd.begin(), fwd.begin() + (long)fg);
+  p->inv_global.assign(inv.begin() + (long)ig, inv.end());
+  if (!to_program(fwd, fg, fwd.size(), p->fwd_lds) || !to_program(inv, 0, ig, p->inv_lds))
+    return set_error(CFFT_ERROR_SIZE, "transform needs more passes than the device program holds");
+  p->chunk_log2_fwd = p->fwd_lds.npass ? p->fwd_lds.log2n : log2n;
+  p->chunk_log2_inv = p->inv_lds.npass ? p->inv_lds.log2n : log2n;
+
+  // tables
+  std::vector<cfft_host::C64> tw = cfft_host::twiddle_table(n);
+  CFFT_HIP(hipMalloc(&p->d_tw, n * sizeof(dc64)));
+  CFFT_HIP(hipMemcpy(p->d_tw, tw.data(), n * sizeof(dc64), hipMemcpyHostToDevice));
+  const unsigned nbits = log2n, bbits = ilog2(p->base_n);
+  std::vector<uint32_t> freq(n);
+  p->pos_of_freq.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    p->pos_of_freq[i] = (uint32_t)cfft_host::freq_position(nbits, bbits, i);
+    freq[i] = (uint32_t)cfft_host::position_freq(nbits, bbits, i);
+  }
+  CFFT_HIP(hipMalloc(&p->d_freq, n * sizeof(uint32_t)));
+  CFFT_HIP(hipMalloc(&p->d_pos, n * sizeof(uint32_t)));
+  CFFT_HIP(hipMemcpy(p->d_freq, freq.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  CFFT_HIP(hipMemcpy(p->d_pos, p->pos_of_freq.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  if (p->kind == cfft_plan_s::UNORDERED) {
+    const size_t N = 2 * n;
+    std::vector<cfft_host::C64> tws = cfft_host::twist_table(N), utw = cfft_host::untwist_table(N);
+    CFFT_HIP(hipMalloc(&p->d_twist, n * sizeof(dc64)));
+    CFFT_HIP(hipMalloc(&p->d_untwist, n * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(p->d_twist, tws.data(), n * sizeof(dc64), hipMemcpyHostToDevice));
+    CFFT_HIP(hipMemcpy(p->d_untwist, utw.data(), n * sizeof(dc64), hipMemcpyHostToDevice));
+  }
+  return CFFT_SUCCESS;
+}
+
+int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, bool fwd,
+                  hipStream_t s) {
+  if (!p) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (p->kind == cfft_plan_s::FFT128)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "an fft128 plan takes split double-double buffers");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!buf) return set_error(CFFT_ERROR_NULL_POINTER, "buffer is null");
+  if (stride < p->n) {
+    char m[160];
+    snprintf(m, sizeof m, "assertion `left == right` failed\n  left: %zu\n right: %zu", p->n, stride);
+    return set_error(CFFT_ERROR_LENGTH_MISMATCH, m);
+  }
+  const uint32_t log2n = ilog2(p->n);
+  dc64 *d = reinterpret_cast<dc64 *>(buf);
+  if (fwd) {
+    for (const Pass &P : p->fwd_global)
+      CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
+    if (p->fwd_lds.npass)
+      CFFT_HIP(cfft_launch::fft_lds(p->fwd_lds, d, batch << (log2n - p->chunk_log2_fwd),
+                                    log2n - p->chunk_log2_fwd, stride, p->d_tw, s));
+  } else {
+    if (p->inv_lds.npass)
+      CFFT_HIP(cfft_launch::fft_lds(p->inv_lds, d, batch << (log2n - p->chunk_log2_inv),
+                                    log2n - p->chunk_log2_inv, stride, p->d_tw, s));
+    for (const Pass &P : p->inv_global)
+      CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
+  }
+  return CFFT_SUCCESS;
+}
+
+int run_transform_host(const cfft_plan_s *p, double *buf, size_t batch, bool fwd) {
+  if (!p) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!buf) return set_error(CFFT_ERROR_NULL_POINTER, "buffer is null");
+  const size_t bytes = p->n * batch * sizeof(dc64);
+  void *d = nullptr;
+  CFFT_HIP(hipMalloc(&d, bytes));
+  int rc = hip_error(hipMemcpy(d, buf, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+  if (rc == CFFT_SUCCESS) rc = run_transform(p, d, batch, p->n, fwd, nullptr);
+  if (rc == CFFT_SUCCESS) rc = hip_error(hipMemcpy(buf, d, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+  (void)hipFree(d);
+  return rc;
+}
+
+// ---- Method::Measure: time candidate plans on the device ----
+double time_plan(cfft_plan_s *p, double seconds) {
+  const size_t n = p->n;
+  const size_t target_bytes = (size_t)32 << 20;
+  size_t batch = std::max<size_t>(1, target_bytes / (n * sizeof(dc64)));
+  void *d = nullptr;
+  if (hipMalloc(&d, batch * n * sizeof(dc64)) != hipSuccess) return 1e30;
+  (void)hipMemset(d, 0, batch * n * sizeof(dc64));
+  hipEvent_t a, b;
+  (void)hipEventCreate(&a);
+  (void)hipEventCreate(&b);
+  // warm up
+  run_transform(p, d, batch, n, true, nullptr);
+  (void)hipDeviceSynchronize();
+  double best = 1e30;
+  auto t0 = std::chrono::steady_clock::now();
+  int reps = 0;
+  do {
+    (void)hipEventRecord(a, nullptr);
+    run_transform(p, d, batch, n, true, nullptr);
+    (void)hipEventRecord(b, nullptr);
+    (void)hipEventSynchronize(b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, a, b);
+    best = std::min(best, (double)ms / (double)batch);
+    ++reps;
+  } while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < seconds &&
+           reps < 1000);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(d);
+  return best;
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, size_t>, std::pair<int, size_t>> g_tune_cache;
+
+int create_f64(cfft_plan_s::Kind kind, size_t n, int algo, size_t base_n, cfft_plan *out) {
+  auto *p = new (std::nothrow) cfft_plan_s();
+  if (!p) return set_error(CFFT_ERROR_OUT_OF_MEMORY, "plan allocation failed");
+  p->kind = kind;
+  p->n = n;
+  p->base_n = base_n;
+  p->algo = algo;
+  (void)hipGetDevice(&p->device);
+  int rc = build_f64_plan(p);
+  if (rc != CFFT_SUCCESS) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return CFFT_SUCCESS;
+}
+
+int measure(cfft_plan_s::Kind kind, size_t n, double seconds, int *algo, size_t *base_n) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find({dev, (int)kind, n});
+    if (it != g_tune_cache.end()) {
+      *algo = it->second.first;
+      *base_n = it->second.second;
+      return CFFT_SUCCESS;
+    }
+  }
+  std::vector<size_t> bases;
+  if (kind == cfft_plan_s::ORDERED || n <= 256) {
+    bases.push_back(n);
+  } else {
+    for (size_t b : {(size_t)512, (size_t)1024})
+      if (b <= n) bases.push_back(b);
+  }
+  const double per = std::max(1e-3, seconds);
+  double best = 1e30;
+  int best_algo = CFFT_DIF4;
+  size_t best_base = bases.front();
+  for (size_t b : bases) {
+    for (int a = 0; a < 8; ++a) {
+      cfft_plan p0 = nullptr;
+      if (create_f64(kind, n, a, b, &p0) != CFFT_SUCCESS) continue;
+      const double t = time_plan(p0, per);
+      delete p0;
+      if (t < best) {
+        best = t;
+        best_algo = a;
+        best_base = b;
+      }
+    }
+  }
+  *algo = best_algo;
+  *base_n = best_base;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune_cache[{dev, (int)kind, n}] = {best_algo, best_base};
+  return CFFT_SUCCESS;
+}
+
+void put_varint(std::vector<uint8_t> &o, uint64_t v) {
+  // bincode 1.3 VarintEncoding (options()): < 251 one byte, then tags
+  // 251 (u16), 252 (u32), 253 (u64), little endian
+  if (v < 251) {
+    o.push_back((uint8_t)v);
+  } else if (v <= 0xffff) {
+    o.push_back(251);
+    for (int i = 0; i < 2; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+  } else if (v <= 0xffffffffull) {
+    o.push_back(252);
+    for (int i = 0; i < 4; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+  } else {
+    o.push_back(253);
+    for (int i = 0; i < 8; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+  }
+}
+
+}  // namespace
+
+int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
+                          size_t batch, hipStream_t s);
+
+extern "C" {
+
+const char *cfft_status_string(int status) {
+  switch (status) {
+    case CFFT_SUCCESS: return "success";
+    case CFFT_ERROR_NOT_POWER_OF_TWO: return "size is not a power of two";
+    case CFFT_ERROR_SIZE: return "size out of the supported range";
+    case CFFT_ERROR_INVALID_BASE: return "invalid base size";
+    case CFFT_ERROR_INVALID_ALGO: return "invalid algorithm";
+    case CFFT_ERROR_NULL_POINTER: return "null pointer";
+    case CFFT_ERROR_LENGTH_MISMATCH: return "buffer length does not match the plan";
+    case CFFT_ERROR_WRONG_PLAN_KIND: return "operation not supported by this plan kind";
+    case CFFT_ERROR_OUT_OF_MEMORY: return "out of memory";
+    case CFFT_ERROR_DEVICE: return "device error";
+    case CFFT_ERROR_INVALID_ARGUMENT: return "invalid argument";
+    case CFFT_ERROR_SIZE_OVERFLOW: return "size overflow";
+    case CFFT_ERROR_DESERIALIZE: return "malformed serialized buffer";
+    case CFFT_ERROR_NO_DEVICE: return "no device";
+    case CFFT_ERROR_BUSY: return "busy";
+    default: return "unknown status";
+  }
+}
+
+const char *cfft_last_error_message(void) { return cfft_internal::g_last_error.c_str(); }
+int cfft_abi_version(void) { return CFFT_ABI_VERSION; }
+
+// ---- device plumbing ----
+int cfft_device_count(int *count) {
+  if (!count) return set_error(CFFT_ERROR_NULL_POINTER, "count is null");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) {
+    *count = 0;
+    return hip_error(e, "hipGetDeviceCount");
+  }
+  *count = c;
+  return CFFT_SUCCESS;
+}
+int cfft_set_device(int device) { return hip_error(hipSetDevice(device), "hipSetDevice"); }
+int cfft_device_alloc(size_t bytes, void **ptr) {
+  if (!ptr) return set_error(CFFT_ERROR_NULL_POINTER, "ptr is null");
+  *ptr = nullptr;
+  if (bytes == 0) return CFFT_SUCCESS;
+  return hip_error(hipMalloc(ptr, bytes), "hipMalloc");
+}
+int cfft_device_free(void *ptr) { return ptr ? hip_error(hipFree(ptr), "hipFree") : CFFT_SUCCESS; }
+int cfft_device_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  size_t f = 0, t = 0;
+  CFFT_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return CFFT_SUCCESS;
+}
+int cfft_memcpy_htod(void *dst, const void *src, size_t bytes, cfft_stream_t stream) {
+  if (bytes == 0) return CFFT_SUCCESS;
+  return hip_error(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "hipMemcpyAsync");
+}
+int cfft_memcpy_dtoh(void *dst, const void *src, size_t bytes, cfft_stream_t stream) {
+  if (bytes == 0) return CFFT_SUCCESS;
+  CFFT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return hip_error(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+int cfft_memcpy_dtod(void *dst, const void *src, size_t bytes, cfft_stream_t stream) {
+  if (bytes == 0) return CFFT_SUCCESS;
+  return hip_error(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream), "hipMemcpyAsync");
+}
+int cfft_memset(void *dst, int value, size_t bytes, cfft_stream_t stream) {
+  if (bytes == 0) return CFFT_SUCCESS;
+  return hip_error(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+}
+int cfft_stream_create(cfft_stream_t *stream) {
+  if (!stream) return set_error(CFFT_ERROR_NULL_POINTER, "stream is null");
+  hipStream_t s = nullptr;
+  CFFT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return CFFT_SUCCESS;
+}
+int cfft_stream_destroy(cfft_stream_t stream) {
+  return stream ? hip_error(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy") : CFFT_SUCCESS;
+}
+int cfft_stream_synchronize(cfft_stream_t stream) {
+  return hip_error(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+int cfft_device_synchronize(void) { return hip_error(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+int cfft_event_create(cfft_event_t *ev) {
+  if (!ev) return set_error(CFFT_ERROR_NULL_POINTER, "event is null");
+  hipEvent_t e = nullptr;
+  CFFT_HIP(hipEventCreate(&e));
+  *ev = e;
+  return CFFT_SUCCESS;
+}
+int cfft_event_destroy(cfft_event_t ev) {
+  return ev ? hip_error(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy") : CFFT_SUCCESS;
+}
+int cfft_event_record(cfft_event_t ev, cfft_stream_t stream) {
+  return hip_error(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream), "hipEventRecord");
+}
+int cfft_event_synchronize(cfft_event_t ev) {
+  return hip_error(hipEventSynchronize((hipEvent_t)ev), "hipEventSynchronize");
+}
+int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms) {
+  if (!ms) return set_error(CFFT_ERROR_NULL_POINTER, "ms is null");
+  return hip_error(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+// ---- plans ----
+int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seconds,
+                             cfft_plan *out) {
+  if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");
+  *out = nullptr;
+  if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
+  if (ilog2(n) >= 11) return set_error(CFFT_ERROR_SIZE, "assertion failed: n.trailing_zeros() < 11");
+  size_t base = n;
+  if (method == CFFT_METHOD_MEASURE) {
+    int rc = measure(cfft_plan_s::ORDERED, n, measure_seconds, &algo, &base);
+    if (rc) return rc;
+  } else if (method != CFFT_METHOD_USER_PROVIDED) {
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown method");
+  }
+  if (algo_radix(algo) == 0) return set_error(CFFT_ERROR_INVALID_ALGO, "unknown FftAlgo");
+  return create_f64(cfft_plan_s::ORDERED, n, algo, n, out);
+}
+
+int cfft_unordered_plan_create(size_t n, int method, int base_algo, size_t base_n,
+                               double measure_seconds, cfft_plan *out) {
+  if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");
+  *out = nullptr;
+  if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
+  if (ilog2(n) > 30) return set_error(CFFT_ERROR_SIZE, "fft size too large for the device plan");
+  if (method == CFFT_METHOD_MEASURE) {
+    int rc = measure(cfft_plan_s::UNORDERED, n, measure_seconds, &base_algo, &base_n);
+    if (rc) return rc;
+  } else if (method == CFFT_METHOD_USER_PROVIDED) {
+    if (!is_pow2(base_n)) return set_error(CFFT_ERROR_INVALID_BASE, "assertion failed: base_n.is_power_of_two()");
+    if (base_n > n) return set_error(CFFT_ERROR_INVALID_BASE, "assertion failed: base_n <= n");
+    if (base_n != n && base_n < 32) return set_error(CFFT_ERROR_INVALID_BASE, "assertion failed: base_n >= 32");
+    if (ilog2(base_n) > 10) return set_error(CFFT_ERROR_INVALID_BASE, "assertion failed: base_n.trailing_zeros() <= 10");
+  } else {
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown method");
+  }
+  if (algo_radix(base_algo) == 0) return set_error(CFFT_ERROR_INVALID_ALGO, "unknown FftAlgo");
+  return create_f64(cfft_plan_s::UNORDERED, n, base_algo, base_n, out);
+}
+
+int cfft_plan_destroy(cfft_plan plan) {
+  delete plan;
+  return CFFT_SUCCESS;
+}
+
+int cfft_plan_fft_size(cfft_plan plan, size_t *n) {
+  if (!plan || !n) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  *n = plan->n;
+  return CFFT_SUCCESS;
+}
+int cfft_plan_algo(cfft_plan plan, int *algo, size_t *base_n) {
+  if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (plan->kind == cfft_plan_s::FFT128)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "fft128 plans have no algorithm choice");
+  if (algo) *algo = plan->algo;
+  if (base_n) *base_n = plan->base_n;
+  return CFFT_SUCCESS;
+}
+int cfft_plan_fft_scratch(cfft_plan plan, size_t *bytes, size_t *align) {
+  if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  const size_t elems = plan->kind == cfft_plan_s::ORDERED ? plan->n
+                       : plan->kind == cfft_plan_s::UNORDERED ? plan->base_n
+                                                               : 0;
+  if (bytes) *bytes = elems * sizeof(dc64);
+  if (align) *align = 128;  // aligned_vec::CACHELINE_ALIGN on x86-64
+  return CFFT_SUCCESS;
+}
+int cfft_plan_permutation(cfft_plan plan, size_t *idx) {
+  if (!plan || !idx) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind == cfft_plan_s::FFT128)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "fft128 output order is bit-reversed");
+  for (size_t i = 0; i < plan->n; ++i) idx[i] = plan->pos_of_freq[i];
+  return CFFT_SUCCESS;
+}
+
+// ---- f64 transforms ----
+int cfft_fwd(cfft_plan plan, void *buf, size_t batch, size_t stride, cfft_stream_t stream) {
+  return run_transform(plan, buf, batch, stride, true, (hipStream_t)stream);
+}
+int cfft_inv(cfft_plan plan, void *buf, size_t batch, size_t stride, cfft_stream_t stream) {
+  return run_transform(plan, buf, batch, stride, false, (hipStream_t)stream);
+}
+int cfft_fwd_host(cfft_plan plan, double *buf, size_t batch) { return run_transform_host(plan, buf, batch, true); }
+int cfft_inv_host(cfft_plan plan, double *buf, size_t batch) { return run_transform_host(plan, buf, batch, false); }
+
+int cfft_fwd_monomial(cfft_plan plan, size_t degree, void *buf, cfft_stream_t stream) {
+  if (!plan || !buf) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind != cfft_plan_s::UNORDERED)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "fwd_monomial is a method of unordered::Plan");
+  if (degree >= plan->n) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "assertion failed: degree < this.fft_size()");
+  return hip_error(cfft_launch::monomial(reinterpret_cast<dc64 *>(buf), ilog2(plan->n), degree,
+                                         plan->d_freq, plan->d_tw, (hipStream_t)stream),
+                   "monomial");
+}
+int cfft_fwd_monomial_host(cfft_plan plan, size_t degree, double *buf) {
+  if (!plan || !buf) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  void *d = nullptr;
+  CFFT_HIP(hipMalloc(&d, plan->n * sizeof(dc64)));
+  int rc = cfft_fwd_monomial(plan, degree, d, nullptr);
+  if (rc == CFFT_SUCCESS) rc = hip_error(hipMemcpy(buf, d, plan->n * sizeof(dc64), hipMemcpyDeviceToHost), "hipMemcpy");
+  (void)hipFree(d);
+  return rc;
+}
+
+// ---- serde ----
+int cfft_fourier_to_standard(cfft_plan plan, const void *permuted, void *standard, size_t batch,
+                             cfft_stream_t stream) {
+  if (!plan || !permuted || !standard) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind == cfft_plan_s::FFT128) return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "f64 plans only");
+  return hip_error(cfft_launch::gather(reinterpret_cast<const dc64 *>(permuted),
+                                       reinterpret_cast<dc64 *>(standard), plan->d_pos,
+                                       (uint32_t)plan->n, batch, (hipStream_t)stream),
+                   "gather");
+}
+int cfft_fourier_from_standard(cfft_plan plan, const void *standard, void *permuted,
+                               size_t batch, cfft_stream_t stream) {
+  if (!plan || !permuted || !standard) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind == cfft_plan_s::FFT128) return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "f64 plans only");
+  return hip_error(cfft_launch::scatter(reinterpret_cast<const dc64 *>(standard),
+                                        reinterpret_cast<dc64 *>(permuted), plan->d_pos,
+                                        (uint32_t)plan->n, batch, (hipStream_t)stream),
+                   "scatter");
+}
+int cfft_serialize_fourier_buffer(cfft_plan plan, const double *buf, uint8_t *out, size_t out_cap,
+                                  size_t *out_len) {
+  if (!plan || !buf || !out_len) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind == cfft_plan_s::FFT128) return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "f64 plans only");
+  std::vector<uint8_t> o;
+  put_varint(o, plan->n);
+  const size_t need = o.size() + plan->n * 16;
+  *out_len = need;
+  if (!out) return CFFT_SUCCESS;
+  if (out_cap < need) return set_error(CFFT_ERROR_SIZE_OVERFLOW, "output buffer too small");
+  std::memcpy(out, o.data(), o.size());
+  uint8_t *w = out + o.size();
+  for (size_t i = 0; i < plan->n; ++i) {
+    const double *z = buf + 2 * (size_t)plan->pos_of_freq[i];
+    std::memcpy(w, z, 16);  // (re, im) little-endian f64
+    w += 16;
+  }
+  return CFFT_SUCCESS;
+}
+int cfft_deserialize_fourier_buffer(cfft_plan plan, const uint8_t *bytes, size_t len, double *buf) {
+  if (!plan || !bytes || !buf) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
+  if (plan->kind == cfft_plan_s::FFT128) return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "f64 plans only");
+  if (len < 1) return set_error(CFFT_ERROR_DESERIALIZE, "unexpected end of input");
+  uint64_t count = 0;
+  size_t off = 0;
+  const uint8_t tag = bytes[0];
+  int w = tag < 251 ? 0 : tag == 251 ? 2 : tag == 252 ? 4 : tag == 253 ? 8 : -1;
+  if (w < 0) return set_error(CFFT_ERROR_DESERIALIZE, "invalid varint tag");
+  if (w == 0) {
+    count = tag;
+    off = 1;
+  } else {
+    if (len < (size_t)(1 + w)) return set_error(CFFT_ERROR_DESERIALIZE, "unexpected end of input");
+    for (int i = 0; i < w; ++i) count |= (uint64_t)bytes[1 + i] << (8 * i);
+    off = 1 + (size_t)w;
+  }
+  if (len < off + count * 16) return set_error(CFFT_ERROR_DESERIALIZE, "unexpected end of input");
+  if (count != plan->n) {
+    char m[160];
+    snprintf(m, sizeof m, "invalid length %llu, expected a sequence of %zu 64-bit complex numbers",
+             (unsigned long long)count, plan->n);
+    return set_error(CFFT_ERROR_DESERIALIZE, m);
+  }
+  for (size_t i = 0; i < plan->n; ++i)
+    std::memcpy(buf + 2 * (size_t)plan->pos_of_freq[i], bytes + off + 16 * i, 16);
+  return CFFT_SUCCESS;
+}
+
+// ---- fused negacyclic product ----
+int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t *b, uint64_t *c,
+                                size_t batch, cfft_stream_t stream) {
+  if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (plan->kind != cfft_plan_s::UNORDERED)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "the negacyclic product runs on an unordered plan");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!a || !b || !c) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  if (plan->n < 2 || plan->fwd_full.npass == 0)
+    return set_error(CFFT_ERROR_SIZE, "negacyclic product needs 2 <= fft_size <= 8192");
+  return cfft_polymul_dispatch(plan, a, b, c, batch, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// generic fused product (the specialised kernels hook in here)
+int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
+                          size_t batch, hipStream_t s) {
+  return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
+                                                p->d_twist, p->d_untwist, s),
+                   "polymul");
+}

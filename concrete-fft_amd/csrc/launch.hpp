@@ -1,0 +1,45 @@
+// launch.hpp -- host-side launchers of the gfx950 kernels (C++ linkage,
+// internal to libcfft_amd.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "program.hpp"
+
+namespace cfft_launch {
+
+struct dc64 {
+  double re, im;
+};
+
+// Largest transform chunk held in one workgroup's LDS (points of 16 B).
+constexpr uint32_t kLdsMaxLog2 = 13;
+
+// Runs `prog` on `ntrans` chunks of 2^prog.log2n points; chunk t lives at
+// data + (t >> chunks_log2) * stride + (t & (2^chunks_log2 - 1)) * 2^log2n.
+hipError_t fft_lds(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,
+                   uint32_t chunks_log2, uint64_t stride, const dc64 *tw, hipStream_t s);
+
+// One DEPTH_FWD / DEPTH_INV pass straight over global memory (levels larger
+// than an LDS chunk).
+hipError_t depth_global(const cfft_prog::Pass &P, dc64 *data, uint64_t batch, uint32_t log2n,
+                        uint64_t stride, const dc64 *tw, hipStream_t s);
+
+// buf[b*stride + i] = T[(freq[i] * degree) & (n-1)]
+hipError_t monomial(dc64 *buf, uint32_t log2n, uint64_t degree, const uint32_t *freq,
+                    const dc64 *tw, hipStream_t s);
+
+// dst[b*n + i] = src[b*n + idx[i]]   /   dst[b*n + idx[i]] = src[b*n + i]
+hipError_t gather(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, uint64_t batch,
+                  hipStream_t s);
+hipError_t scatter(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, uint64_t batch,
+                   hipStream_t s);
+
+// Fused negacyclic product over the generic pass executor.
+hipError_t polymul_generic(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                           const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                           const dc64 *tw, const dc64 *twist, const dc64 *untwist,
+                           hipStream_t s);
+
+}  // namespace cfft_launch

@@ -1,0 +1,45 @@
+// plan.hpp -- internal plan object behind the opaque cfft_plan handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "launch.hpp"
+#include "program.hpp"
+
+struct cfft_plan_s {
+  enum Kind { ORDERED = 0, UNORDERED = 1, FFT128 = 2 };
+  Kind kind = UNORDERED;
+  size_t n = 0;       // fft_size
+  size_t base_n = 0;  // internal ordered size (== n for ordered plans)
+  int algo = 0;       // (base) algorithm, cfft_algo
+  int device = 0;
+
+  // f64 path
+  cfft_launch::dc64 *d_tw = nullptr;  // T[j] = (cos, sin)(-2 pi j / n)
+  uint32_t *d_freq = nullptr;         // position -> frequency
+  uint32_t *d_pos = nullptr;          // frequency -> position
+  std::vector<uint32_t> pos_of_freq;  // host copy of d_pos
+  // LDS program and the depth passes too large for one workgroup
+  cfft_prog::Program fwd_lds{}, inv_lds{};
+  uint32_t chunk_log2_fwd = 0, chunk_log2_inv = 0;
+  std::vector<cfft_prog::Pass> fwd_global, inv_global;  // fwd: before, inv: after
+  // whole-transform programs (n <= LDS chunk) used by the fused kernels
+  cfft_prog::Program fwd_full{}, inv_full{};
+  // negacyclic product tables (N = 2n)
+  cfft_launch::dc64 *d_twist = nullptr, *d_untwist = nullptr;
+
+  // fft128
+  double *d_tw128 = nullptr;  // 4 x n: re0 | re1 | im0 | im1
+
+  ~cfft_plan_s();
+};
+
+namespace cfft_internal {
+int set_error(int status, const std::string &msg);
+int hip_error(hipError_t e, const char *where);
+}  // namespace cfft_internal

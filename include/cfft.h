@@ -1,0 +1,230 @@
+/*
+ * cfft.h -- C ABI of the MI355X (gfx950) concrete-fft engine.
+ *
+ * This is the drop-in boundary for zama-ai/concrete-fft v0.5.1.  Every entry
+ * point names the reference item it replaces (path:line in the reference
+ * crate).  Conventions:
+ *   - every function returns an int status (cfft_status); none aborts.  A
+ *     Rust shim maps a non-zero status to `panic!` with the message of
+ *     cfft_last_error_message(), which reproduces the reference's panic text.
+ *   - plans are immutable after creation and may be shared between threads
+ *     (the reference's Plan is Clone + Send + Sync, src/ordered.rs:186,
+ *     src/unordered.rs:495).
+ *   - complex buffers are the reference's `[c64]` layout: interleaved
+ *     {re, im} IEEE-754 doubles (num_complex::Complex64, src/lib.rs:84).
+ *     A batch is `batch` transforms of `fft_size` values, transform b
+ *     starting at element b*stride (stride >= fft_size, in c64 units).
+ *   - device entry points take device pointers and a stream (hipStream_t,
+ *     NULL = default stream) and are stream ordered; *_host entry points take
+ *     host pointers, copy through the device and synchronise.
+ *   - no torch types, no C++ types: plain pointers and sizes only.
+ */
+#ifndef CFFT_H
+#define CFFT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CFFT_ABI_VERSION 1
+
+typedef struct cfft_plan_s *cfft_plan;
+typedef void *cfft_stream_t; /* hipStream_t */
+typedef void *cfft_event_t;  /* hipEvent_t */
+
+typedef enum {
+  CFFT_SUCCESS = 0,
+  CFFT_ERROR_NOT_POWER_OF_TWO = 1, /* assert!(n.is_power_of_two()) */
+  CFFT_ERROR_SIZE = 2,             /* n out of the plan's supported range */
+  CFFT_ERROR_INVALID_BASE = 3,     /* unordered base_n rules, src/unordered.rs:664-669 */
+  CFFT_ERROR_INVALID_ALGO = 4,
+  CFFT_ERROR_NULL_POINTER = 5,
+  CFFT_ERROR_LENGTH_MISMATCH = 6,  /* assert_eq!(self.fft_size(), buf.len()) */
+  CFFT_ERROR_WRONG_PLAN_KIND = 7,  /* e.g. fwd_monomial on an ordered plan */
+  CFFT_ERROR_OUT_OF_MEMORY = 8,
+  CFFT_ERROR_DEVICE = 9,           /* HIP runtime error */
+  CFFT_ERROR_INVALID_ARGUMENT = 10,
+  CFFT_ERROR_SIZE_OVERFLOW = 11,   /* dyn_stack::SizeOverflow */
+  CFFT_ERROR_DESERIALIZE = 12,     /* serde invalid_length / malformed stream */
+  CFFT_ERROR_NO_DEVICE = 13,
+  CFFT_ERROR_BUSY = 14             /* server admission refused */
+} cfft_status;
+
+/* ordered::FftAlgo, src/ordered.rs:26-45 (same discriminant order). */
+typedef enum {
+  CFFT_DIF2 = 0,
+  CFFT_DIT2 = 1,
+  CFFT_DIF4 = 2,
+  CFFT_DIT4 = 3,
+  CFFT_DIF8 = 4,
+  CFFT_DIT8 = 5,
+  CFFT_DIF16 = 6,
+  CFFT_DIT16 = 7
+} cfft_algo;
+
+/* ordered::Method (src/ordered.rs:48-59) / unordered::Method
+ * (src/unordered.rs:526-537).  MEASURE is served by this library's GPU
+ * autotuner (it times the candidate plans on the device). */
+typedef enum { CFFT_METHOD_USER_PROVIDED = 0, CFFT_METHOD_MEASURE = 1 } cfft_method;
+
+/* ---- status / diagnostics ---- */
+const char *cfft_status_string(int status);
+/* Thread-local text of the last failure on this thread, worded like the
+ * reference's panic message for the same contract violation. */
+const char *cfft_last_error_message(void);
+int cfft_abi_version(void);
+
+/* ---- device plumbing (no reference counterpart: the reference has no
+ *      device boundary; these let a caller without a GPU runtime binding
+ *      drive the library) ---- */
+int cfft_device_count(int *count);
+int cfft_set_device(int device);
+int cfft_device_alloc(size_t bytes, void **ptr);
+int cfft_device_free(void *ptr);
+int cfft_device_mem_info(size_t *free_bytes, size_t *total_bytes);
+int cfft_memcpy_htod(void *dst, const void *src, size_t bytes, cfft_stream_t stream);
+int cfft_memcpy_dtoh(void *dst, const void *src, size_t bytes, cfft_stream_t stream);
+int cfft_memcpy_dtod(void *dst, const void *src, size_t bytes, cfft_stream_t stream);
+int cfft_memset(void *dst, int value, size_t bytes, cfft_stream_t stream);
+int cfft_stream_create(cfft_stream_t *stream);
+int cfft_stream_destroy(cfft_stream_t stream);
+int cfft_stream_synchronize(cfft_stream_t stream);
+int cfft_device_synchronize(void);
+int cfft_event_create(cfft_event_t *ev);
+int cfft_event_destroy(cfft_event_t ev);
+int cfft_event_record(cfft_event_t ev, cfft_stream_t stream);
+int cfft_event_synchronize(cfft_event_t ev);
+int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms);
+
+/* ---- plans ---- */
+/* ordered::Plan::new(n, method), src/ordered.rs:242-278: n a power of two,
+ * n <= 2^10.  `algo` is used for USER_PROVIDED; MEASURE times all 8 algos on
+ * the device for about `measure_seconds` each (Method::Measure(Duration)). */
+int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seconds,
+                             cfft_plan *out);
+/* unordered::Plan::new(n, method), src/unordered.rs:659-747: n a power of
+ * two; base_n a power of two with base_n <= n, base_n <= 2^10 and
+ * base_n >= 32 unless base_n == n.  MEASURE picks (base_algo, base_n) on the
+ * device. */
+int cfft_unordered_plan_create(size_t n, int method, int base_algo, size_t base_n,
+                               double measure_seconds, cfft_plan *out);
+/* fft128::Plan::new(n), src/fft128/mod.rs:1864-1885: n a power of two,
+ * n >= 32. */
+int cfft128_plan_create(size_t n, cfft_plan *out);
+int cfft_plan_destroy(cfft_plan plan);
+
+/* Plan::fft_size, src/ordered.rs:290 / src/unordered.rs:765 /
+ * src/fft128/mod.rs:1896 */
+int cfft_plan_fft_size(cfft_plan plan, size_t *n);
+/* ordered Plan::algo (src/ordered.rs:304) -> (algo, n);
+ * unordered Plan::algo (src/unordered.rs:783) -> (base_algo, base_n) */
+int cfft_plan_algo(cfft_plan plan, int *algo, size_t *base_n);
+/* Plan::fft_scratch (src/ordered.rs:320, src/unordered.rs:798): bytes and
+ * alignment of the scratch the reference asks its caller for (n resp. base_n
+ * c64, cache-line aligned).  The device path needs none (it works in LDS);
+ * the figure is reported for drop-in fidelity. */
+int cfft_plan_fft_scratch(cfft_plan plan, size_t *bytes, size_t *align);
+/* Position of frequency i in the plan's permuted Fourier order:
+ * idx[i] = bit_rev_twice(log2 n, log2 base_n, i) (src/unordered.rs:1046-1051).
+ * Identity for ordered plans. */
+int cfft_plan_permutation(cfft_plan plan, size_t *idx);
+
+/* ---- f64 transforms ---- */
+/* ordered Plan::fwd / Plan::inv (src/ordered.rs:342-373) and unordered
+ * Plan::fwd / Plan::inv (src/unordered.rs:826-839, :927-940): in place,
+ * unnormalised; unordered fwd leaves the result in the plan's permuted order
+ * and inv expects it.  Results are bit-identical to the reference's f64
+ * path for the same plan (algo, base_n). */
+int cfft_fwd(cfft_plan plan, void *buf, size_t batch, size_t stride, cfft_stream_t stream);
+int cfft_inv(cfft_plan plan, void *buf, size_t batch, size_t stride, cfft_stream_t stream);
+int cfft_fwd_host(cfft_plan plan, double *buf, size_t batch);
+int cfft_inv_host(cfft_plan plan, double *buf, size_t batch);
+/* unordered Plan::fwd_monomial(degree, buf), src/unordered.rs:844-900:
+ * forward transform of X^degree in the plan's permuted order. */
+int cfft_fwd_monomial(cfft_plan plan, size_t degree, void *buf, cfft_stream_t stream);
+int cfft_fwd_monomial_host(cfft_plan plan, size_t degree, double *buf);
+
+/* ---- Fourier-buffer serde (feature "serde", src/unordered.rs:942-1036) ---- */
+/* Gather a permuted-order buffer into standard order (what
+ * serialize_fourier_buffer emits element by element) / scatter standard order
+ * into the plan's permuted order (deserialize_fourier_buffer).  Device. */
+int cfft_fourier_to_standard(cfft_plan plan, const void *permuted, void *standard, size_t batch,
+                             cfft_stream_t stream);
+int cfft_fourier_from_standard(cfft_plan plan, const void *standard, void *permuted,
+                               size_t batch, cfft_stream_t stream);
+/* Byte stream of serialize_fourier_buffer through bincode 1.3 `options()`
+ * (varint sequence length, then n x (re, im) little-endian f64), as in
+ * test_serde (src/unordered.rs:9411-9466).  Host buffers.  With out == NULL
+ * only *out_len is set. */
+int cfft_serialize_fourier_buffer(cfft_plan plan, const double *buf, uint8_t *out, size_t out_cap,
+                                  size_t *out_len);
+int cfft_deserialize_fourier_buffer(cfft_plan plan, const uint8_t *bytes, size_t len,
+                                    double *buf);
+
+/* ---- fft128 (feature "fft128", src/fft128/mod.rs) ---- */
+/* fft128::Plan::fwd / inv (src/fft128/mod.rs:1905-1960): split double-double
+ * arrays re0, re1, im0, im1 (hi/lo words), each batch x n, in place; fwd
+ * leaves bit-reversed order, inv expects it; unnormalised.  Device. */
+int cfft128_fwd(cfft_plan plan, double *re0, double *re1, double *im0, double *im1, size_t batch,
+                cfft_stream_t stream);
+int cfft128_inv(cfft_plan plan, double *re0, double *re1, double *im0, double *im1, size_t batch,
+                cfft_stream_t stream);
+int cfft128_fwd_host(cfft_plan plan, double *re0, double *re1, double *im0, double *im1,
+                     size_t batch);
+int cfft128_inv_host(cfft_plan plan, double *re0, double *re1, double *im0, double *im1,
+                     size_t batch);
+/* #[doc(hidden)] free functions negacyclic_{fwd,inv}_fft{,_scalar,_avxfma}
+ * (src/fft128/mod.rs:351-1102), with caller-provided twiddles (device
+ * pointers, n each).  mul_variant 0 reproduces the scalar path's
+ * double-double product, 1 the AVX2/AVX-512 path's fused form. */
+int cfft128_negacyclic_fwd_fft(size_t n, double *re0, double *re1, double *im0, double *im1,
+                               const double *tw_re0, const double *tw_re1, const double *tw_im0,
+                               const double *tw_im1, size_t batch, int mul_variant,
+                               cfft_stream_t stream);
+int cfft128_negacyclic_inv_fft(size_t n, double *re0, double *re1, double *im0, double *im1,
+                               const double *tw_re0, const double *tw_re1, const double *tw_im0,
+                               const double *tw_im1, size_t batch, int mul_variant,
+                               cfft_stream_t stream);
+/* init_negacyclic_twiddles (src/fft128/mod.rs:1804-1828), host arrays of n. */
+int cfft128_init_negacyclic_twiddles(size_t n, double *re0, double *re1, double *im0,
+                                     double *im1);
+
+/* ---- fused negacyclic products (caller-side in the reference: TFHE-rs
+ *      builds them on unordered::Plan; the fold layout follows
+ *      src/fft128/mod.rs:1996-2016) ---- */
+/* C = A * B mod (X^N + 1), N = 2 * fft_size(plan), coefficients on the
+ * 2^-64 torus: A u64 torus coefficients, B signed integer coefficients
+ * (gadget digits), C u64 torus.  One kernel: fold, twist, two forward
+ * transforms, pointwise product, inverse, untwist, 1/n and torus rounding.
+ * Bit-identical to the reference-based CPU product for the same plan. */
+int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t *b, uint64_t *c,
+                                size_t batch, cfft_stream_t stream);
+/* Exact C = A * B mod (X^N + 1, 2^64) for u64 A, B: balanced 16-bit limbs,
+ * one f64 negacyclic transform per limb, per-weight Fourier products,
+ * integer rounding and carries.  `margin` (optional, device, batch doubles)
+ * receives max |x - round(x)| over the rounded products of each polynomial:
+ * the result is exact while it stays < 0.5. */
+int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                           double *margin, size_t batch, cfft_stream_t stream);
+
+/* ---- external product (TFHE GGSW x GLWE; caller-side in the reference) ---- */
+/* Transform a GGSW key of k1*levels rows x k1 columns of u64 torus
+ * polynomials (N each) into the Fourier domain of `plan` (device, in place
+ * layout [row][col][n] c64). */
+int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels, void *key_fourier,
+                         cfft_stream_t stream);
+/* out (k1 polys) = GGSW(key_fourier) (x) GLWE(glwe): gadget decomposition
+ * (levels x base 2^base_log, balanced digits), forward transforms, Fourier
+ * MAC against the resident key, inverse, torus rounding.  Device, batch of
+ * independent GLWEs sharing one key. */
+int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int levels,
+                          int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
+                          cfft_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+"""GPU parity of the fused negacyclic product: bit-identical to the oracle's
+reference-based CPU product, and its torus error against the exact u64
+schoolbook equals the CPU path's (it is the same computation)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_inputs(seed, batch, N):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, 2**64, size=(batch, N), dtype=np.uint64)
+    b = rng.integers(-128, 128, size=(batch, N), dtype=np.int64)
+    return a, b
+
+
+def torus_err(c, exact):
+    return np.abs((c - exact).view(np.int64)).max()
+
+
+@pytest.mark.parametrize("N,algo,base_n", [(2048, 6, 1024), (2048, 2, 32), (2048, 7, 512),
+                                           (1024, 4, 512), (4096, 3, 1024), (256, 0, 128),
+                                           (64, 6, 32), (16384, 6, 1024)])
+def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    batch = 33
+    a, b = rand_inputs(N + algo, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo(algo), base_n))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, algo, base_n)
+    assert np.array_equal(c, ref), (N, algo, base_n, int(np.sum(c != ref)))
+
+
+def test_polymul_torus_error_equals_cpu(cfft, oracle):
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N, batch = 2048, 16
+    a, b = rand_inputs(2, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    exact = oracle.schoolbook_u64(a, b.view(np.uint64))
+    cpu = oracle.polymul_f64(a, b, 6, 1024)
+    e_gpu, e_cpu = torus_err(c, exact), torus_err(cpu, exact)
+    assert e_gpu <= e_cpu
+    # f64 FFT product of a 64-bit torus polynomial: error well below 2^-24 of the torus
+    assert e_gpu < 2**40
+
+
+def test_polymul_exact_on_coarse_grid(cfft, oracle):
+    # torus inputs on the 2^-20 grid (A = a * 2^44) and digits in [-4, 4): the
+    # exact product lies on the same grid, and the f64 error (|C| <= 2^11 torus
+    # units, relative error ~2^-50) is far below half a grid step, so rounding
+    # the f64 product to the grid recovers the schoolbook result exactly
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N, batch = 1024, 8
+    rng = np.random.default_rng(11)
+    a = (rng.integers(-2**19, 2**19, size=(batch, N), dtype=np.int64) << 44).view(np.uint64)
+    b = rng.integers(-4, 4, size=(batch, N), dtype=np.int64)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif8, 512))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    exact = oracle.schoolbook_u64(a, b.view(np.uint64))
+    rounded = ((c + np.uint64(1 << 43)) >> np.uint64(44)) << np.uint64(44)
+    assert np.array_equal(rounded, exact)
+    assert torus_err(c, exact) < 2**40
+
+
+def test_polymul_monomial_shift(cfft, oracle):
+    # B = X^k: the product is the negacyclic shift of A; A lies on the 2^-31
+    # grid and the f64 error (~2^13 units of 2^-64) is far below half a step
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    rng = np.random.default_rng(12)
+    a = (rng.integers(-2**30, 2**30, size=(1, N), dtype=np.int64) << 33).view(np.uint64)
+    for k in (0, 1, 5, N - 1):
+        b = np.zeros((1, N), dtype=np.int64)
+        b[0, k] = 1
+        plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+        c = products.negacyclic_polymul_u64_host(plan, a, b)
+        shifted = np.concatenate([(-a[0, N - k:].view(np.int64)).view(np.uint64), a[0, :N - k]])
+        rounded = ((c[0] + np.uint64(1 << 32)) >> np.uint64(33)) << np.uint64(33)
+        assert np.array_equal(rounded, shifted), k

@@ -1,0 +1,142 @@
+"""Pins of the oracle's f64 path (CPU only).
+
+Each check is chosen so a plausible restatement mistake fails it: the golden
+vector is the reference's own bit-exact output (src/unordered.rs:1176-1395);
+numpy's pocketfft and a brute-force mpmath DFT are independent libraries; the
+roundtrip, permutation and serde checks are the reference's own test
+relations (src/ordered.rs:389-451, src/unordered.rs:1073-1172, :9411-9466).
+"""
+import os
+
+import mpmath
+import numpy as np
+import pytest
+
+from oracle_lib import load_oracle
+
+ALGOS = list(range(8))  # Dif2, Dit2, Dif4, Dit4, Dif8, Dit8, Dif16, Dit16
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "unordered_n2048_dif4_b32.npy")
+
+
+@pytest.fixture(scope="module")
+def O():
+    return load_oracle()
+
+
+def golden_input(O):
+    v = O.stdrng_f64(0, 2 * 2048)
+    return v.view(np.complex128).copy()
+
+
+def test_rng_first_sum(O):
+    # the golden output's element 0 is the sum of the inputs (src/unordered.rs:1202-1205)
+    z = golden_input(O)
+    assert abs(z.sum() - complex(1028.0259662650005, 1018.8306666971462)) < 1e-9
+
+
+def test_golden_vector_bit_exact(O):
+    g = np.load(GOLDEN, allow_pickle=False)
+    z = O.unordered_fwd(golden_input(O), 2, 32)  # Dif4, base_n = 32
+    assert np.array_equal(z.view(np.float64), g.view(np.float64)), \
+        "oracle differs from the reference golden vector at %d places" % int(np.sum(z != g))
+
+
+def test_golden_vector_detects_planted_mistake(O):
+    # a wrong base size (different permutation) or algorithm breaks bit-exactness
+    g = np.load(GOLDEN, allow_pickle=False)
+    z = golden_input(O)
+    assert not np.array_equal(O.unordered_fwd(z, 2, 64), g)
+    assert not np.array_equal(O.unordered_fwd(z, 6, 32), g)
+
+
+def test_sincospi64_within_one_ulp(O):
+    mpmath.mp.prec = 120
+    xs = [-2.0 * j / 4096 for j in range(4096)] + [j / 4096 for j in range(2048)]
+    rng = np.random.default_rng(7)
+    xs += list(rng.uniform(-1, 1, 500))
+    for x in xs:
+        s, c = O.sincospi64(float(x))
+        es = mpmath.sinpi(mpmath.mpf(x))
+        ec = mpmath.cospi(mpmath.mpf(x))
+        for got, exp in ((s, es), (c, ec)):
+            err = abs(mpmath.mpf(got) - exp)
+            ulp = np.spacing(abs(float(exp))) if float(exp) != 0 else 2.0 ** -1074
+            assert err <= ulp, (x, got, float(exp))
+
+
+def test_sincospi64_symmetry_bitwise(O):
+    # the device twiddle table relies on sincospi(-x) = (-s, c) bit for bit
+    for j in range(1, 2048):
+        x = 2.0 * j / 2048
+        s1, c1 = O.sincospi64(x)
+        s2, c2 = O.sincospi64(-x)
+        assert s1 == -s2 and c1 == c2, x
+
+
+@pytest.mark.parametrize("algo", ALGOS)
+def test_ordered_vs_numpy(O, algo):
+    rng = np.random.default_rng(100 + algo)
+    for e in range(1, 11):
+        n = 1 << e
+        x = rng.random(n) + 1j * rng.random(n)
+        y = O.ordered_fwd(x, algo)
+        assert np.max(np.abs(y - np.fft.fft(x))) < 1e-12, n
+        z = O.ordered_inv(y, algo) / n
+        assert np.max(np.abs(z - x)) < 1e-14, n
+
+
+@pytest.mark.parametrize("algo", [0, 3, 6])
+def test_ordered_vs_bruteforce_mpmath(O, algo):
+    mpmath.mp.prec = 100
+    rng = np.random.default_rng(5)
+    n = 32
+    x = rng.random(n) + 1j * rng.random(n)
+    y = O.ordered_fwd(x, algo)
+    for p in range(n):
+        acc = mpmath.mpc(0)
+        for q in range(n):
+            acc += mpmath.mpc(x[q].real, x[q].imag) * mpmath.expjpi(mpmath.mpf(-2 * p * q) / n)
+        assert abs(complex(acc) - y[p]) < 1e-13
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("base_n", [32, 64, 256])
+def test_unordered_vs_numpy_through_permutation(O, n, base_n):
+    if base_n > n:
+        pytest.skip("base_n > n is rejected by Plan::new")
+    rng = np.random.default_rng(n + base_n)
+    x = rng.random(n) + 1j * rng.random(n)
+    for algo in (2, 3, 6):
+        z = O.unordered_fwd(x, algo, base_n)
+        target = np.fft.fft(x)
+        nb, bb = n.bit_length() - 1, base_n.bit_length() - 1
+        idx = np.array([O.bit_rev_twice(nb, bb, i) for i in range(n)])
+        assert np.max(np.abs(z[idx] - target)) < 1e-12 * max(1, n / 1024)
+        back = O.unordered_inv(z, algo, base_n) / n
+        assert np.max(np.abs(back - x)) < 1e-12
+
+
+def test_unordered_serde_cross_plan(O):
+    # test_serde: serialise with base 32 (standard order), deserialise into base 64
+    rng = np.random.default_rng(3)
+    for n in (64, 128, 256, 512, 1024):
+        x = rng.random(n) + 1j * rng.random(n)
+        z = O.unordered_fwd(x, 2, 32)
+        nb = n.bit_length() - 1
+        std = z[[O.bit_rev_twice(nb, 5, i) for i in range(n)]]
+        z2 = np.empty_like(z)
+        z2[[O.bit_rev_twice(nb, 6, i) for i in range(n)]] = std
+        back = O.unordered_inv(z2, 2, 64) / n
+        assert np.max(np.abs(back - x)) < 1e-12
+
+
+def test_permutation_closed_form(O):
+    # bit_rev_twice is a permutation, inverse of bit_rev_twice_inv, identity when base_n == n
+    for nb in range(1, 12):
+        for bb in range(0, nb + 1):
+            perm = [O.bit_rev_twice(nb, bb, i) for i in range(1 << nb)]
+            assert sorted(perm) == list(range(1 << nb))
+            for i in range(1 << nb):
+                assert O.L.or_bit_rev_twice_inv(nb, bb, perm[i]) == i
+            if bb == nb:
+                assert perm == list(range(1 << nb))
