@@ -227,6 +227,9 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   }
   const uint32_t log2n = ilog2(p->n);
   dc64 *d = reinterpret_cast<dc64 *>(buf);
+  if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
+    return hip_error(cfft_launch::r16_transform(log2n, fwd, d, batch, stride, p->d_tw, s),
+                     "r16_transform");
   if (fwd) {
     for (const Pass &P : p->fwd_global)
       CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
@@ -669,6 +672,10 @@ int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t
 // generic fused product (the specialised kernels hook in here)
 int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
                           size_t batch, hipStream_t s) {
+  const uint32_t log2n = ilog2(p->n);
+  if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
+    return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw, p->d_twist, s),
+                     "r16_polymul");
   return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
                                                 p->d_twist, p->d_untwist, s),
                    "polymul");

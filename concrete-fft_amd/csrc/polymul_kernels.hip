@@ -1,0 +1,143 @@
+// polymul_kernels.hip -- register-resident kernels for Dif16 plans with
+// base_n == n, 512 <= n <= 4096: plain forward / inverse transforms and the
+// fused negacyclic product (N = 2n).
+#include <hip/hip_runtime.h>
+
+#include "launch.hpp"
+#include "r16_exec.hpp"
+
+using namespace cfft_dev;
+using cfft_launch::dc64;
+
+namespace {
+
+// twist zeta_j and its scaled conjugate conj(zeta_j) / n (exact: n is a power
+// of two), the latter recomputed instead of loaded
+__device__ __forceinline__ c64 untwist_from(c64 z, double inv_n) {
+  return mk(z.re * inv_n, (-z.im) * inv_n);
+}
+
+template <int LOG2N, bool FWD>
+__global__ void __launch_bounds__(R16<LOG2N>::T) k_r16_transform(c64 *__restrict__ data,
+                                                                  uint64_t batch, uint64_t stride,
+                                                                  const c64 *__restrict__ tw) {
+  using S = R16<LOG2N>;
+  __shared__ c64 lds[S::lds_words];
+  const uint32_t l = threadIdx.x;
+  const uint64_t t = blockIdx.x;
+  if (t >= batch) return;
+  c64 *x = data + t * stride;
+  c64 v[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) v[m] = x[l + S::T * m];
+  r16_transform<LOG2N, FWD>(v, lds, l, tw);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) x[l + S::T * S::end_m(s)] = v[s];
+}
+
+template <int LOG2N>
+__global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per_eu(2, 2))) k_r16_polymul(
+    const uint64_t *__restrict__ A, const int64_t *__restrict__ B, uint64_t *__restrict__ C,
+    uint64_t batch, const c64 *__restrict__ tw, const c64 *__restrict__ twist) {
+  using S = R16<LOG2N>;
+  constexpr uint32_t n = S::n;
+  __shared__ c64 lds[S::lds_words];
+  const uint32_t l = threadIdx.x;
+  const double inv_n = 1.0 / (double)n;
+  {
+    const uint64_t poly = blockIdx.x;
+    if (poly >= batch) return;
+    const c64 *twist_i = twist;
+    const uint64_t *a = A + poly * (2 * n);
+    const int64_t *b = B + poly * (2 * n);
+    uint64_t *c = C + poly * (2 * n);
+    c64 fa[16], v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      fa[m] = cmul(twist_i[j], mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+      if ((m & 3) == 3) sched_fence();
+    }
+    r16_transform<LOG2N, true>(fa, lds, l, tw);
+    sched_fence();
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      v[m] = cmul(twist_i[j], mk((double)b[j], (double)b[j + n]));
+      if ((m & 3) == 3) sched_fence();
+    }
+    r16_transform<LOG2N, true>(v, lds, l, tw);
+    sched_fence();
+    c64 z[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
+    r16_end_to_input<LOG2N>(fa, z);
+    r16_transform<LOG2N, false>(z, lds, l, tw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const c64 r = cmul(untwist_from(twist_i[j], inv_n), z[s]);
+      c[j] = from_torus(r.re);
+      c[j + n] = from_torus(r.im);
+      if ((s & 3) == 3) sched_fence();
+    }
+  }
+}
+
+template <int LOG2N>
+hipError_t launch_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                            const dc64 *tw, hipStream_t s) {
+  using S = R16<LOG2N>;
+  const uint32_t grid = (uint32_t)batch;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  if (fwd)
+    hipLaunchKernelGGL((k_r16_transform<LOG2N, true>), dim3(grid), dim3(S::T), 0, s, d, batch,
+                       stride, t);
+  else
+    hipLaunchKernelGGL((k_r16_transform<LOG2N, false>), dim3(grid), dim3(S::T), 0, s, d, batch,
+                       stride, t);
+  return hipGetLastError();
+}
+
+template <int LOG2N>
+hipError_t launch_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                          const dc64 *tw, const dc64 *twist, hipStream_t s) {
+  using S = R16<LOG2N>;
+  const uint32_t grid = (uint32_t)batch;
+  hipLaunchKernelGGL(k_r16_polymul<LOG2N>, dim3(grid), dim3(S::T), 0, s, a, b, c, batch,
+                     reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
+  return hipGetLastError();
+}
+
+}  // namespace
+
+namespace cfft_launch {
+
+bool r16_supported(uint32_t log2n) { return log2n >= 9 && log2n <= 12; }
+
+hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                         const dc64 *tw, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  switch (log2n) {
+    case 9: return launch_transform<9>(fwd, data, batch, stride, tw, s);
+    case 10: return launch_transform<10>(fwd, data, batch, stride, tw, s);
+    case 11: return launch_transform<11>(fwd, data, batch, stride, tw, s);
+    case 12: return launch_transform<12>(fwd, data, batch, stride, tw, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
+                       uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  switch (log2n) {
+    case 9: return launch_polymul<9>(a, b, c, batch, tw, twist, s);
+    case 10: return launch_polymul<10>(a, b, c, batch, tw, twist, s);
+    case 11: return launch_polymul<11>(a, b, c, batch, tw, twist, s);
+    case 12: return launch_polymul<12>(a, b, c, batch, tw, twist, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace cfft_launch

@@ -1,0 +1,139 @@
+// r16_exec.hpp -- register-resident schedule of the reference's Dif16 plan
+// for transforms of n = 2^LOG2N points, 9 <= LOG2N <= 12 (base_n == n).
+//
+// The reference's Dif16 recursion (src/dif16.rs:900-920) runs, for these n,
+// two radix-16 Stockham cores (strides 1 and 16) and one end butterfly of
+// radix 2^(LOG2N-8) (src/dif2.rs / dif4.rs / dif8.rs / dif16.rs *_end).
+// With T = n/16 threads per transform:
+//   core s=1   thread l reads points l + T*k        (k < 16), writes 16 l + k
+//   core s=16  thread l reads points l + T*k,  writes q + 256 p + 16 k
+//              (p = l / 16, q = l % 16)
+//   end        thread l reads and writes points l + T*(i + NB*k)
+//              (i < NB = 16 / REND butterflies, k < REND)
+// so every thread's inputs and outputs of a whole transform are the points
+// l + T*m, m < 16 ("lane-linear"): loads, stores, the twist, the pointwise
+// product and the hand-over from a forward to an inverse transform happen in
+// registers, and only the two interior exchanges go through LDS.  Each
+// butterfly and twiddle product is the reference's operation DAG, so results
+// are bit-identical to the generic schedule.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace cfft_dev {
+
+template <bool FWD>
+__device__ __forceinline__ c64 tw_at_impl(const c64 *__restrict__ tw, uint32_t j) {
+  c64 w = tw[j];
+  if (!FWD) w.im = -w.im;
+  return w;
+}
+
+__device__ __forceinline__ uint32_t r16_pad(uint32_t i) { return i + (i >> 4); }
+
+template <int LOG2N>
+struct R16 {
+  static constexpr uint32_t n = 1u << LOG2N;
+  static constexpr uint32_t T = n / 16;
+  static constexpr uint32_t LOG2REND = LOG2N - 8;
+  static constexpr uint32_t REND = 1u << LOG2REND;
+  static constexpr uint32_t NB = 16 / REND;
+  static constexpr uint32_t lds_words = n + n / 16;
+  // point index (minus l, over T) held by register slot s after the end pass:
+  // slot s = i * REND + k holds m = i + NB * k
+  static __device__ __forceinline__ constexpr uint32_t end_m(uint32_t s) {
+    return (s / REND) + NB * (s % REND);
+  }
+};
+
+// Hides a pointer's value from the optimiser so loads through it are not
+// merged across transforms (the three transforms of a product read the same
+// twiddles; keeping them live between transforms costs 120 VGPRs).
+// (an opaque zero offset keeps the pointer's global address space, unlike
+// passing the pointer itself through asm)
+template <class P>
+__device__ __forceinline__ const P *launder(const P *p) {
+  uint32_t zero = 0;
+  asm volatile("" : "+s"(zero));
+  return p + zero;
+}
+
+// Scheduling fence: keeps the machine scheduler from hoisting every load of a
+// transform to its top, which would need far more than 256 VGPRs.
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// v[k] *= w_k for k = 1..15, twiddles fetched four at a time
+template <bool FWD, class Idx>
+__device__ __forceinline__ void twiddle16(c64 *v, const c64 *__restrict__ tw, Idx idx) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    c64 w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) w[e] = tw_at_impl<FWD>(tw, idx(k));
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) v[k] = cmul(w[e], v[k]);
+    }
+    sched_fence();
+  }
+}
+
+template <bool FWD>
+__device__ __forceinline__ c64 tw_at(const c64 *__restrict__ tw, uint32_t j) {
+  c64 w = tw[j];
+  if (!FWD) w.im = -w.im;
+  return w;
+}
+
+// One transform, data in registers in the lane-linear layout:
+// in:  v[m] = point l + T*m
+// out: v[s] = point l + T*end_m(s)
+// `lds` holds lds_words points; the caller's barrier discipline: this
+// function begins with writes to LDS, so any earlier reads of LDS by other
+// threads must be complete (it ends with a barrier after its last read).
+template <int LOG2N, bool FWD>
+__device__ __forceinline__ void r16_transform(c64 *v, c64 *__restrict__ lds, uint32_t l,
+                                              const c64 *__restrict__ tw) {
+  using S = R16<LOG2N>;
+  tw = launder(tw);
+  // ---- core, stride 1: butterfly of points l + T*k, twiddle k*l ----
+  butterfly<16, FWD>(v);
+  sched_fence();
+  twiddle16<FWD>(v, tw, [&](int k) { return (uint32_t)k * l; });
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[r16_pad(16 * l + (uint32_t)k)] = v[k];
+  __syncthreads();
+  // ---- core, stride 16 ----
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = lds[r16_pad(l + S::T * (uint32_t)k)];
+  const uint32_t p = l >> 4, q = l & 15;
+  butterfly<16, FWD>(v);
+  sched_fence();
+  twiddle16<FWD>(v, tw, [&](int k) { return (uint32_t)k * p * 16u; });
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) lds[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
+  __syncthreads();
+  // ---- end butterflies of radix REND at stride n / REND ----
+#pragma unroll
+  for (uint32_t i = 0; i < S::NB; ++i) {
+    c64 *u = v + i * S::REND;
+#pragma unroll
+    for (uint32_t k = 0; k < S::REND; ++k) u[k] = lds[r16_pad(l + S::T * (i + S::NB * k))];
+    butterfly<S::REND, FWD>(u);
+  }
+  __syncthreads();
+}
+
+// Move register slots from the end-pass layout to the input layout.
+template <int LOG2N>
+__device__ __forceinline__ void r16_end_to_input(const c64 *src, c64 *dst) {
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) dst[R16<LOG2N>::end_m(s)] = src[s];
+}
+
+}  // namespace cfft_dev
