@@ -70,6 +70,13 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
          ["-lquadmath", "-lpthread"])
     os.replace(tmp, LIB)
+    # C++ mirror example (links the library through its in-tree rpath)
+    host = os.path.join(HERE, "host")
+    ex = os.path.join(LIBDIR, "example_roundtrip")
+    _run(["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + host,
+          os.path.join(host, "example_roundtrip.cpp"), "-o", ex + ".tmp", "-L" + LIBDIR,
+          "-lcfft_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(ex + ".tmp", ex)
     return LIB
 
 

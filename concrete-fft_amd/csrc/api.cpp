@@ -56,6 +56,7 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_freq) (void)hipFree(d_freq);
   if (d_pos) (void)hipFree(d_pos);
   if (d_twist) (void)hipFree(d_twist);
+  if (d_tw_r16) (void)hipFree(d_tw_r16);
   if (d_untwist) (void)hipFree(d_untwist);
   if (d_tw128) (void)hipFree(d_tw128);
 }
@@ -191,6 +192,16 @@ int build_f64_plan(cfft_plan_s *p) {
   std::vector<cfft_host::C64> tw = cfft_host::twiddle_table(n);
   CFFT_HIP(hipMalloc(&p->d_tw, n * sizeof(dc64)));
   CFFT_HIP(hipMemcpy(p->d_tw, tw.data(), n * sizeof(dc64), hipMemcpyHostToDevice));
+  if (p->algo == CFFT_DIF16 && p->base_n == n && cfft_launch::r16_supported(log2n)) {
+    const size_t T = n / 16, T2 = T / 16;
+    std::vector<cfft_host::C64> blk(16 * T + 16 * T2);
+    for (size_t k = 0; k < 16; ++k) {
+      for (size_t l = 0; l < T; ++l) blk[k * T + l] = tw[(k * l) % n];
+      for (size_t q = 0; q < T2; ++q) blk[16 * T + k * T2 + q] = tw[(16 * k * q) % n];
+    }
+    CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
+  }
   const unsigned nbits = log2n, bbits = ilog2(p->base_n);
   std::vector<uint32_t> freq(n);
   p->pos_of_freq.resize(n);
@@ -228,7 +239,7 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   const uint32_t log2n = ilog2(p->n);
   dc64 *d = reinterpret_cast<dc64 *>(buf);
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
-    return hip_error(cfft_launch::r16_transform(log2n, fwd, d, batch, stride, p->d_tw, s),
+    return hip_error(cfft_launch::r16_transform(log2n, fwd, d, batch, stride, p->d_tw_r16, s),
                      "r16_transform");
   if (fwd) {
     for (const Pass &P : p->fwd_global)
@@ -674,7 +685,7 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
                           size_t batch, hipStream_t s) {
   const uint32_t log2n = ilog2(p->n);
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
-    return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw, p->d_twist, s),
+    return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw_r16, p->d_twist, s),
                      "r16_polymul");
   return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
                                                 p->d_twist, p->d_untwist, s),

@@ -20,7 +20,9 @@
 
 namespace cfft_dev {
 
-struct c64 {
+// 16-byte aligned so LDS and global accesses of one point are single
+// 128-bit operations
+struct alignas(16) c64 {
   double re, im;
 };
 
@@ -226,10 +228,24 @@ __device__ __forceinline__ constexpr int bitrev_digit(int k) {
 __device__ __forceinline__ double torus_to_f64(uint64_t a) {
   return (double)(int64_t)a * 5.421010862427522e-20;  // 2^-64
 }
+// Nearest multiple of 2^-64 of frac(x), as a u64 (fract in [-1/2, 1/2]; the
+// value 2^63 wraps to the same torus element as -2^63).  Reference formula:
+//   fract = x - round(x); v = round(fract * 2^64); (u64)(i64)v.
+// Fast path, same bits: when ulp(x) >= 2^-64 (biased exponent >= 1011),
+// x * 2^64 is an integer, fract * 2^64 = x * 2^64 - round(x) * 2^64 exactly,
+// and the result is x * 2^64 mod 2^64: the mantissa shifted left, negated
+// for negative x.  Otherwise |x| < 2^-11, round(x) = 0 and the formula is
+// applied directly.
 __device__ __forceinline__ uint64_t from_torus(double x) {
-  double fract = x - round(x);
-  double v = round(fract * 18446744073709551616.0);
-  if (v >= 9223372036854775808.0) return 0x8000000000000000ull;
+  const uint64_t bits = (uint64_t)__double_as_longlong(x);
+  const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
+  if (ex >= 1011u) {
+    const uint64_t m = (bits & 0x000fffffffffffffull) | 0x0010000000000000ull;
+    const uint32_t sh = ex - 1011u;
+    const uint64_t r = sh >= 64u ? 0ull : (m << sh);
+    return (bits >> 63) ? (0ull - r) : r;
+  }
+  const double v = round(x * 18446744073709551616.0);
   return (uint64_t)(int64_t)v;
 }
 

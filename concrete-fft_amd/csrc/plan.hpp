@@ -30,6 +30,9 @@ struct cfft_plan_s {
   std::vector<cfft_prog::Pass> fwd_global, inv_global;  // fwd: before, inv: after
   // whole-transform programs (n <= LDS chunk) used by the fused kernels
   cfft_prog::Program fwd_full{}, inv_full{};
+  // r16 (register-resident Dif16) twiddle block: T[k*l] as [k][l] (16 x n/16)
+  // then T[16 k p] as [k][p] (16 x n/256)
+  cfft_launch::dc64 *d_tw_r16 = nullptr;
   // negacyclic product tables (N = 2n)
   cfft_launch::dc64 *d_twist = nullptr, *d_untwist = nullptr;
 

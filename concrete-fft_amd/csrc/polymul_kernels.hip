@@ -6,6 +6,8 @@
 #include "launch.hpp"
 #include "r16_exec.hpp"
 
+#include <cstdlib>
+
 using namespace cfft_dev;
 using cfft_launch::dc64;
 
@@ -52,20 +54,16 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
     const int64_t *b = B + poly * (2 * n);
     uint64_t *c = C + poly * (2 * n);
     c64 fa[16], v[16];
+    // A, B and the twist are fetched together: one memory latency per product
 #pragma unroll
     for (uint32_t m = 0; m < 16; ++m) {
       const uint32_t j = l + S::T * m;
-      fa[m] = cmul(twist_i[j], mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
-      if ((m & 3) == 3) sched_fence();
+      const c64 z = twist_i[j];
+      fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+      v[m] = cmul(z, mk((double)b[j], (double)b[j + n]));
     }
     r16_transform<LOG2N, true>(fa, lds, l, tw);
     sched_fence();
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t j = l + S::T * m;
-      v[m] = cmul(twist_i[j], mk((double)b[j], (double)b[j + n]));
-      if ((m & 3) == 3) sched_fence();
-    }
     r16_transform<LOG2N, true>(v, lds, l, tw);
     sched_fence();
     c64 z[16];
@@ -77,6 +75,61 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
     for (uint32_t s = 0; s < 16; ++s) {
       const uint32_t j = l + S::T * S::end_m(s);
       const c64 r = cmul(untwist_from(twist_i[j], inv_n), z[s]);
+      c[j] = from_torus(r.re);
+      c[j + n] = from_torus(r.im);
+      if ((s & 3) == 3) sched_fence();
+    }
+  }
+}
+
+// Persistent product kernel for n = 1024 (N = 2048): 8 waves per workgroup,
+// each running its own stream of products; the r16 twiddle block is staged
+// once per workgroup in LDS and shared by the waves.
+constexpr uint32_t kPW = 8;  // waves per workgroup
+constexpr uint32_t kTwWords = 16 * 64 + 16 * 4;
+constexpr size_t kPersistLds = (kTwWords + kPW * R16<10>::lds_words) * sizeof(c64);
+
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+                         uint64_t *__restrict__ C, uint64_t batch,
+                         const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
+  using S = R16<10>;
+  constexpr uint32_t n = S::n;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
+  __syncthreads();
+  const uint32_t wave = threadIdx.x >> 6;
+  c64 *x = smem + kTwWords + wave * S::lds_words;
+  const double inv_n = 1.0 / (double)n;
+  for (uint64_t poly = (uint64_t)blockIdx.x * kPW + wave; poly < batch;
+       poly += (uint64_t)gridDim.x * kPW) {
+    // recomputed per product so per-lane addresses are not kept live across
+    // the loop
+    const uint32_t l = launder_u32(threadIdx.x) & 63u;
+    const uint64_t *a = A + poly * (2 * n);
+    const int64_t *b = B + poly * (2 * n);
+    uint64_t *c = C + poly * (2 * n);
+    const c64 *tws = launder(twist);
+    c64 fa[16], v[16];
+    // A, B and the twist are fetched together: one memory latency per product
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      const c64 z = tws[j];
+      fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+      v[m] = cmul(z, mk((double)b[j], (double)b[j + n]));
+    }
+    r16w_transform<true, 2>(fa, v, x, l, ltw);
+    c64 z[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
+    r16_end_to_input<10>(fa, z);
+    r16w_transform<false, 1>(z, z, x, l, ltw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
       c[j] = from_torus(r.re);
       c[j + n] = from_torus(r.im);
       if ((s & 3) == 3) sched_fence();
@@ -100,10 +153,36 @@ hipError_t launch_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t strid
   return hipGetLastError();
 }
 
+hipError_t launch_polymul_persistent(const uint64_t *a, const int64_t *b, uint64_t *c,
+                                    uint64_t batch, const dc64 *tw, const dc64 *twist,
+                                    hipStream_t s) {
+  static int cus = 0;
+  static bool lds_ok = false;
+  if (!lds_ok) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      return hipErrorInvalidDevice;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_persistent),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)kPersistLds);
+    if (e != hipSuccess) return e;
+    lds_ok = true;
+  }
+  const uint64_t want = (batch + kPW - 1) / kPW;
+  const uint32_t grid = (uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus);
+  hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
+                     c, batch, reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
+  return hipGetLastError();
+}
+
 template <int LOG2N>
 hipError_t launch_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                           const dc64 *tw, const dc64 *twist, hipStream_t s) {
   using S = R16<LOG2N>;
+  if (LOG2N == 10 && getenv("CFFT_R16_GRID") == nullptr)
+    return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
   const uint32_t grid = (uint32_t)batch;
   hipLaunchKernelGGL(k_r16_polymul<LOG2N>, dim3(grid), dim3(S::T), 0, s, a, b, c, batch,
                      reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));

@@ -58,6 +58,11 @@ __device__ __forceinline__ const P *launder(const P *p) {
   return p + zero;
 }
 
+__device__ __forceinline__ uint32_t launder_u32(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // Scheduling fence: keeps the machine scheduler from hoisting every load of a
 // transform to its top, which would need far more than 256 VGPRs.
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
@@ -95,6 +100,10 @@ __device__ __forceinline__ c64 tw_at(const c64 *__restrict__ tw, uint32_t j) {
 // `lds` holds lds_words points; the caller's barrier discipline: this
 // function begins with writes to LDS, so any earlier reads of LDS by other
 // threads must be complete (it ends with a barrier after its last read).
+// `tw` is the plan's r16 twiddle block: rows k = 0..15 of T entries holding
+// T[k*l] for lane l (pass 1), followed by rows k = 0..15 of T/16 entries
+// holding T[16*k*p] (pass 2).  Same values as the full table, laid out so
+// every twiddle load of a wave is contiguous.
 template <int LOG2N, bool FWD>
 __device__ __forceinline__ void r16_transform(c64 *v, c64 *__restrict__ lds, uint32_t l,
                                               const c64 *__restrict__ tw) {
@@ -103,7 +112,7 @@ __device__ __forceinline__ void r16_transform(c64 *v, c64 *__restrict__ lds, uin
   // ---- core, stride 1: butterfly of points l + T*k, twiddle k*l ----
   butterfly<16, FWD>(v);
   sched_fence();
-  twiddle16<FWD>(v, tw, [&](int k) { return (uint32_t)k * l; });
+  twiddle16<FWD>(v, tw, [&](int k) { return (uint32_t)k * S::T + l; });
 #pragma unroll
   for (int k = 0; k < 16; ++k) lds[r16_pad(16 * l + (uint32_t)k)] = v[k];
   __syncthreads();
@@ -113,7 +122,7 @@ __device__ __forceinline__ void r16_transform(c64 *v, c64 *__restrict__ lds, uin
   const uint32_t p = l >> 4, q = l & 15;
   butterfly<16, FWD>(v);
   sched_fence();
-  twiddle16<FWD>(v, tw, [&](int k) { return (uint32_t)k * p * 16u; });
+  twiddle16<FWD>(v, tw + 16 * S::T, [&](int k) { return (uint32_t)k * (S::T / 16) + p; });
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < 16; ++k) lds[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
@@ -134,6 +143,118 @@ template <int LOG2N>
 __device__ __forceinline__ void r16_end_to_input(const c64 *src, c64 *dst) {
 #pragma unroll
   for (uint32_t s = 0; s < 16; ++s) dst[R16<LOG2N>::end_m(s)] = src[s];
+}
+
+}  // namespace cfft_dev
+
+namespace cfft_dev {
+
+// ---- wave-local variant: one transform per wavefront (T == 64) ----
+//
+// LDS instructions of one wavefront execute in issue order, so an exchange
+// among the lanes of a single wave needs no s_barrier; the fence only stops
+// the compiler from moving LDS accesses across it.
+__device__ __forceinline__ void wave_lds_fence() {
+  __asm__ __volatile__("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <bool FWD>
+__device__ __forceinline__ void twiddle16_x2(c64 *va, c64 *vb, const c64 *tw_row0,
+                                             uint32_t row_stride, uint32_t col) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    c64 w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) w[e] = tw_at_impl<FWD>(tw_row0, (uint32_t)k * row_stride + col);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) {
+        va[k] = cmul(w[e], va[k]);
+        vb[k] = cmul(w[e], vb[k]);
+      }
+    }
+    sched_fence();
+  }
+}
+
+template <bool FWD>
+__device__ __forceinline__ void twiddle16_x1(c64 *v, const c64 *tw_row0, uint32_t row_stride,
+                                             uint32_t col) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    c64 w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) w[e] = tw_at_impl<FWD>(tw_row0, (uint32_t)k * row_stride + col);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) v[k] = cmul(w[e], v[k]);
+    }
+    sched_fence();
+  }
+}
+
+// n = 1024 points per wave.  `tw` points at the r16 twiddle block in LDS;
+// `x` is this wave's exchange buffer (R16<10>::lds_words points).
+// Lane-linear layout as r16_transform.  NV = 1 or 2 transforms sharing
+// every twiddle read (va, vb).
+template <bool FWD, int NV>
+__device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                               const c64 *tw) {
+  using S = R16<10>;
+  // core, stride 1
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
+  sched_fence();
+  if (NV == 2) twiddle16_x2<FWD>(va, vb, tw, S::T, l);
+  else twiddle16_x1<FWD>(va, tw, S::T, l);
+  // exchange 1
+  const uint32_t p = l >> 4, q = l & 15;
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    c64 *v = t == 0 ? va : vb;
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[r16_pad(16 * l + (uint32_t)k)] = v[k];
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
+  }
+  // core, stride 16
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
+  sched_fence();
+  if (NV == 2) twiddle16_x2<FWD>(va, vb, tw + 16 * S::T, S::T / 16, p);
+  else twiddle16_x1<FWD>(va, tw + 16 * S::T, S::T / 16, p);
+  // exchange 2
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    c64 *v = t == 0 ? va : vb;
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t i = 0; i < S::NB; ++i)
+#pragma unroll
+      for (uint32_t k = 0; k < S::REND; ++k)
+        v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
+  }
+  wave_lds_fence();
+  // end butterflies (radix 4)
+#pragma unroll
+  for (uint32_t i = 0; i < S::NB; ++i) {
+    butterfly<S::REND, FWD>(va + i * S::REND);
+    if (NV == 2) butterfly<S::REND, FWD>(vb + i * S::REND);
+  }
 }
 
 }  // namespace cfft_dev

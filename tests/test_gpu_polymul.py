@@ -82,3 +82,20 @@ def test_polymul_monomial_shift(cfft, oracle):
         shifted = np.concatenate([(-a[0, N - k:].view(np.int64)).view(np.uint64), a[0, :N - k]])
         rounded = ((c[0] + np.uint64(1 << 32)) >> np.uint64(33)) << np.uint64(33)
         assert np.array_equal(rounded, shifted), k
+
+
+def test_polymul_small_outputs_bit_exact(cfft, oracle):
+    # products below 2^-11 in magnitude take the rounding branch of the torus
+    # conversion (their ulp is finer than 2^-64); an all-zero row gives exact 0
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N, batch = 2048, 8
+    rng = np.random.default_rng(21)
+    a = rng.integers(-2**40, 2**40, size=(batch, N), dtype=np.int64).view(np.uint64)
+    b = rng.integers(-3, 4, size=(batch, N), dtype=np.int64)
+    a[0] = 0
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, 6, 1024)
+    assert np.array_equal(c, ref), int(np.sum(c != ref))
+    assert not c[0].any()
