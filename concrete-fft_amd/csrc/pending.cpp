@@ -10,10 +10,6 @@ using cfft_internal::set_error;
 
 extern "C" {
 
-int cfft_exact_polymul_u64(cfft_plan, const uint64_t *, const uint64_t *, uint64_t *, double *,
-                           size_t, cfft_stream_t) {
-  return set_error(CFFT_ERROR_INVALID_ARGUMENT, "exact product: not built in this library");
-}
 int cfft_ggsw_to_fourier(cfft_plan, const uint64_t *, int, int, void *, cfft_stream_t) {
   return set_error(CFFT_ERROR_INVALID_ARGUMENT, "external product: not built in this library");
 }

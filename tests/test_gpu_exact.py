@@ -1,0 +1,60 @@
+"""Exact u64 negacyclic product (16-bit limbs, f64 transforms, integer
+rounding and carries): bit-exact against the u64 schoolbook oracle, for any
+plan, with the reported rounding margin below 1/2."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+M = 2**64
+
+
+@pytest.mark.parametrize("N,algo,base_n", [(4, 6, 2), (64, 6, 32), (256, 2, 128),
+                                           (2048, 6, 1024), (2048, 3, 32), (4096, 6, 2048),
+                                           (4096, 0, 512)])
+def test_exact_bit_exact_random(cfft, oracle, N, algo, base_n):
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(N + algo)
+    batch = 6
+    a = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    b = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo(algo), base_n))
+    c, margin = products.exact_polymul_u64_host(plan, a, b, with_margin=True)
+    exact = oracle.schoolbook_u64(a, b)
+    assert np.array_equal(c, exact), (N, int(np.sum(c != exact)))
+    assert margin.max() < 0.5 and margin.min() >= 0.0
+    print("N=%d max rounding margin %.3g" % (N, margin.max()))
+
+
+def test_exact_edge_values(cfft, oracle):
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    rows = []
+    ones = np.full(N, M - 1, dtype=np.uint64)        # all 0xffff...ffff
+    zeros = np.zeros(N, dtype=np.uint64)
+    top = np.full(N, 1 << 63, dtype=np.uint64)
+    alt = np.where(np.arange(N) % 2 == 0, np.uint64(0x8000800080008000), np.uint64(0x7fff7fff7fff7fff))
+    mono = zeros.copy()
+    mono[N - 1] = 1
+    for x, y in [(ones, ones), (zeros, ones), (top, top), (alt, ones), (alt, alt), (mono, alt),
+                 (ones, mono)]:
+        rows.append((x, y))
+    a = np.stack([r[0] for r in rows])
+    b = np.stack([r[1] for r in rows])
+    c = products.exact_polymul_u64_host(plan, a, b)
+    assert np.array_equal(c, oracle.schoolbook_u64(a, b))
+
+
+def test_exact_independent_of_plan(cfft):
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 1024
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, M, (3, N), dtype=np.uint64)
+    b = rng.integers(0, M, (3, N), dtype=np.uint64)
+    outs = [products.exact_polymul_u64_host(Plan(N // 2, Method.UserProvided(FftAlgo(al), bn)), a, b)
+            for al, bn in [(6, 512), (1, 32), (4, 128)]]
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])

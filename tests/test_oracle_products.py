@@ -1,0 +1,91 @@
+"""CPU pins of the product oracles: the u64 negacyclic schoolbook (against
+Python integers and the X^N = -1 shift invariant), the oracle's f64 product
+(torus error bound, exactness on a coarse grid), and the gadget
+decomposition (recomposition identity and digit range)."""
+import numpy as np
+import pytest
+
+from oracle_lib import load_oracle
+
+M64 = (1 << 64) - 1
+
+
+@pytest.fixture(scope="module")
+def O():
+    return load_oracle()
+
+
+def schoolbook_py(a, b):
+    N = len(a)
+    c = [0] * N
+    for i in range(N):
+        for j in range(N):
+            k = i + j
+            if k < N:
+                c[k] += int(a[i]) * int(b[j])
+            else:
+                c[k - N] -= int(a[i]) * int(b[j])
+    return [x & M64 for x in c]
+
+
+@pytest.mark.parametrize("N", [1, 2, 4, 8, 32])
+def test_schoolbook_vs_python_integers(O, N):
+    rng = np.random.default_rng(N)
+    a = rng.integers(0, 2**64, N, dtype=np.uint64)
+    b = rng.integers(0, 2**64, N, dtype=np.uint64)
+    got = O.schoolbook_u64(a[None], b[None])[0]
+    assert [int(x) for x in got] == schoolbook_py(a, b)
+
+
+def test_schoolbook_shift_invariant(O):
+    # multiplying by X^k rotates with a sign flip on wrap (X^N = -1)
+    N = 256
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2**64, N, dtype=np.uint64)
+    for k in (0, 1, 77, N - 1):
+        xk = np.zeros(N, dtype=np.uint64)
+        xk[k] = 1
+        got = O.schoolbook_u64(a[None], xk[None])[0]
+        want = np.concatenate([(-(a[N - k:].view(np.int64))).view(np.uint64), a[:N - k]])
+        assert np.array_equal(got, want)
+
+
+def test_f64_product_error_bound_and_grid_exactness(O):
+    N, batch = 2048, 4
+    rng = np.random.default_rng(4)
+    a = rng.integers(0, 2**64, (batch, N), dtype=np.uint64)
+    b = rng.integers(-128, 128, (batch, N), dtype=np.int64)
+    c = O.polymul_f64(a, b, 6, 1024)
+    exact = O.schoolbook_u64(a, b.view(np.uint64))
+    err = np.abs((c - exact).view(np.int64)).max()
+    assert 0 < err < 2**32  # f64 FFT noise of a 64-bit torus product
+    # coarse grid: exact after rounding to the grid
+    a2 = (rng.integers(-2**19, 2**19, (batch, N), dtype=np.int64) << 44).view(np.uint64)
+    b2 = rng.integers(-4, 4, (batch, N), dtype=np.int64)
+    c2 = O.polymul_f64(a2, b2, 2, 32)
+    ex2 = O.schoolbook_u64(a2, b2.view(np.uint64))
+    assert np.array_equal(((c2 + np.uint64(1 << 43)) >> np.uint64(44)) << np.uint64(44), ex2)
+
+
+def test_from_torus_closed_form(O):
+    for x, want in [(0.0, 0), (0.25, 1 << 62), (-0.25, (1 << 64) - (1 << 62)), (0.5, 1 << 63),
+                    (-0.5, 1 << 63), (3.75, 3 << 62), (2.0 ** -64, 1), (-(2.0 ** -64), M64),
+                    (2.0 ** -66, 0)]:
+        assert O.L.or_from_torus(x) == want, x
+
+
+@pytest.mark.parametrize("levels,base_log", [(3, 8), (2, 10), (4, 6), (1, 16)])
+def test_gadget_decomposition(O, levels, base_log):
+    rng = np.random.default_rng(levels * 100 + base_log)
+    x = rng.integers(0, 2**64, 4096, dtype=np.uint64)
+    x[:4] = [0, M64, 1 << 63, (1 << 63) - 1]
+    d = O.gadget_decompose(x, levels, base_log)
+    half = 1 << (base_log - 1)
+    assert d.min() >= -half and d.max() < half
+    rep = levels * base_log
+    for j in range(x.size):
+        v = int(x[j])
+        # closest multiple of 2^(64 - rep), ties up
+        rounded = ((v >> (64 - rep - 1)) + 1) >> 1 << (64 - rep) if rep < 64 else v
+        recomposed = sum(int(d[i, j]) << (64 - base_log * (i + 1)) for i in range(levels))
+        assert recomposed & M64 == rounded & M64, (j, v)
