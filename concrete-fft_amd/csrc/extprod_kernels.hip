@@ -1,0 +1,235 @@
+// extprod_kernels.hip -- TFHE external product (GGSW (x) GLWE) on a resident
+// Fourier-domain key, and the key's transform into that domain.
+//
+// Conventions (caller-side in the reference; same as the negacyclic product):
+// twist zeta_j, fold (re = x_j, im = x_{j+n}), the plan's unordered forward
+// transform, cmul with the key / torus operand first, untwist conj(zeta)/n,
+// torus rounding.  Gadget decomposition: round x to the nearest multiple of
+// 2^(64 - levels*base_log) (ties up), then balanced digits in
+// [-2^(B-1), 2^(B-1)) from the least significant level up, carrying.
+// Row r = p * levels + i is the level-i digit polynomial of GLWE poly p
+// (level 0 most significant).
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <set>
+
+#include "launch.hpp"
+#include "stockham_exec.hpp"
+
+using namespace cfft_dev;
+using cfft_launch::dc64;
+using cfft_prog::Program;
+
+namespace {
+
+__device__ __forceinline__ int64_t gadget_digit(uint64_t x, uint32_t level, uint32_t levels,
+                                                uint32_t base_log) {
+  const uint32_t rep = levels * base_log;
+  const uint32_t shift = 64 - rep;
+  uint64_t st = shift == 0 ? x : (x >> shift) + ((x >> (shift - 1)) & 1u);
+  if (rep < 64) st &= ((uint64_t)1 << rep) - 1;
+  const uint64_t mask = ((uint64_t)1 << base_log) - 1;
+  const int64_t half = (int64_t)1 << (base_log - 1);
+  int64_t d = 0;
+  for (int32_t i = (int32_t)levels - 1; i >= (int32_t)level; --i) {
+    d = (int64_t)(st & mask);
+    st >>= base_log;
+    if (d >= half) {
+      d -= (int64_t)1 << base_log;
+      st += 1;
+    }
+  }
+  return d;
+}
+
+// key: npolys u64 torus polynomials -> Fourier (plan order), one per transform
+template <int E>
+__global__ void __launch_bounds__(512) k_torus_to_fourier(const uint64_t *__restrict__ key,
+                                                          c64 *__restrict__ out, uint64_t npolys,
+                                                          const c64 *__restrict__ tw,
+                                                          const c64 *__restrict__ twist,
+                                                          Program fwd) {
+  extern __shared__ c64 smem[];
+  const uint32_t n = 1u << fwd.log2n;
+  const uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t t = (uint64_t)blockIdx.x * G + g;
+  const bool active = t < npolys;
+  c64 *lds = smem + g * lds_words(n);
+  const uint64_t *src = key + t * 2ull * n;
+  auto load = [&](uint32_t j) -> c64 {
+    if (!active) return mk(0.0, 0.0);
+    return cmul(twist[j], mk(torus_to_f64(src[j]), torus_to_f64(src[j + n])));
+  };
+  c64 *dst = out + t * n;
+  auto store = [&](uint32_t pos, c64 v) {
+    if (active) dst[pos] = v;
+  };
+  run_program<E>(fwd, lds, lt, T, tw, load, store);
+}
+
+// One GLWE per transform group.  LDS: work buffer + k1 accumulators.
+template <int E>
+__global__ void __launch_bounds__(512) k_extprod(const c64 *__restrict__ keyf,
+                                                 const uint64_t *__restrict__ glwe,
+                                                 uint64_t *__restrict__ out, uint64_t batch,
+                                                 uint32_t k1, uint32_t levels, uint32_t base_log,
+                                                 const c64 *__restrict__ tw,
+                                                 const c64 *__restrict__ twist,
+                                                 const c64 *__restrict__ untwist, Program fwd,
+                                                 Program inv) {
+  extern __shared__ c64 smem[];
+  const uint32_t n = 1u << fwd.log2n;
+  const uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t p = (uint64_t)blockIdx.x * G + g;
+  const bool active = p < batch;
+  const uint32_t words = lds_words(n);
+  c64 *lds = smem + g * (1 + k1) * words;
+  c64 *acc = lds + words;
+  for (uint32_t i = lt; i < k1 * words; i += T) acc[i] = mk(0.0, 0.0);
+  __syncthreads();
+  const uint64_t *in = glwe + p * (uint64_t)k1 * 2ull * n;
+  const uint32_t rows = k1 * levels;
+  for (uint32_t r = 0; r < rows; ++r) {
+    const uint32_t poly = r / levels, lvl = r % levels;
+    const uint64_t *src = in + (uint64_t)poly * 2ull * n;
+    auto load = [&](uint32_t j) -> c64 {
+      if (!active) return mk(0.0, 0.0);
+      return cmul(twist[j], mk((double)gadget_digit(src[j], lvl, levels, base_log),
+                               (double)gadget_digit(src[j + n], lvl, levels, base_log)));
+    };
+    const c64 *krow = keyf + (uint64_t)r * k1 * n;
+    auto store = [&](uint32_t pos, c64 v) {
+      for (uint32_t c = 0; c < k1; ++c) {
+        c64 &a = acc[c * words + lds_pad(pos)];
+        const c64 pr = cmul(krow[(uint64_t)c * n + pos], v);
+        a = mk(a.re + pr.re, a.im + pr.im);
+      }
+    };
+    run_program<E>(fwd, lds, lt, T, tw, load, store);
+  }
+  for (uint32_t c = 0; c < k1; ++c) {
+    const c64 *ac = acc + c * words;
+    auto load = [&](uint32_t pos) -> c64 { return ac[lds_pad(pos)]; };
+    uint64_t *o = out + (p * k1 + c) * 2ull * n;
+    auto store = [&](uint32_t j, c64 v) {
+      if (!active) return;
+      const c64 r = cmul(untwist[j], v);
+      o[j] = from_torus(r.re);
+      o[j + n] = from_torus(r.im);
+    };
+    run_program<E>(inv, lds, lt, T, tw, load, store);
+  }
+}
+
+hipError_t set_lds(const void *kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::set<const void *> done;
+  if (bytes <= 64 * 1024) return hipSuccess;
+  std::lock_guard<std::mutex> gd(mu);
+  if (done.count(kernel)) return hipSuccess;
+  int dev = 0, maxb = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return e;
+  if ((size_t)maxb < bytes) return hipErrorInvalidValue;
+  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, maxb);
+  if (e != hipSuccess) return e;
+  done.insert(kernel);
+  return hipSuccess;
+}
+
+struct Geom {
+  int E;
+  uint32_t T, G, threads;
+};
+Geom geom(uint32_t log2n, uint32_t words_per_group_limit, uint32_t groups_words) {
+  Geom g;
+  const uint32_t n = 1u << log2n;
+  g.E = n >= 16 ? 16 : (int)n;
+  g.T = n / (uint32_t)g.E;
+  g.G = n >= 2048 ? 1 : 2048 / n;
+  if (g.T * g.G > 512) g.G = 512 / g.T > 0 ? 512 / g.T : 1;
+  while (g.G > 1 && (size_t)g.G * groups_words * sizeof(c64) > words_per_group_limit) g.G /= 2;
+  g.threads = g.T * g.G;
+  return g;
+}
+
+template <int E>
+hipError_t launch_key(const Program &fwd, const uint64_t *key, c64 *out, uint64_t npolys,
+                      const c64 *tw, const c64 *twist, hipStream_t s) {
+  const uint32_t words = lds_words(1u << fwd.log2n);
+  const Geom g = geom(fwd.log2n, 64 * 1024, words);
+  const size_t lds = (size_t)g.G * words * sizeof(c64);
+  hipError_t e = set_lds(reinterpret_cast<const void *>(k_torus_to_fourier<E>), lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_torus_to_fourier<E>, dim3((uint32_t)((npolys + g.G - 1) / g.G)),
+                     dim3(g.threads), lds, s, key, out, npolys, tw, twist, fwd);
+  return hipGetLastError();
+}
+
+template <int E>
+hipError_t launch_ext(const Program &fwd, const Program &inv, const c64 *keyf,
+                      const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
+                      uint32_t levels, uint32_t base_log, const c64 *tw, const c64 *twist,
+                      const c64 *untwist, hipStream_t s) {
+  const uint32_t words = lds_words(1u << fwd.log2n);
+  const uint32_t gw = (1 + k1) * words;
+  const Geom g = geom(fwd.log2n, 80 * 1024, gw);
+  const size_t lds = (size_t)g.G * gw * sizeof(c64);
+  hipError_t e = set_lds(reinterpret_cast<const void *>(k_extprod<E>), lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_extprod<E>, dim3((uint32_t)((batch + g.G - 1) / g.G)), dim3(g.threads),
+                     lds, s, keyf, glwe, out, batch, k1, levels, base_log, tw, twist, untwist,
+                     fwd, inv);
+  return hipGetLastError();
+}
+
+int pick_e(uint32_t log2n) {
+  const uint32_t n = 1u << log2n;
+  return n >= 16 ? 16 : (int)n;
+}
+
+}  // namespace
+
+namespace cfft_launch {
+
+size_t external_product_lds_bytes(uint32_t log2n, uint32_t k1) {
+  return (size_t)(1 + k1) * lds_words(1u << log2n) * sizeof(c64);
+}
+
+hipError_t torus_to_fourier(const Program &fwd, const uint64_t *key, dc64 *out, uint64_t npolys,
+                            const dc64 *tw, const dc64 *twist, hipStream_t s) {
+  if (npolys == 0) return hipSuccess;
+  c64 *o = reinterpret_cast<c64 *>(out);
+  const c64 *t = reinterpret_cast<const c64 *>(tw), *z = reinterpret_cast<const c64 *>(twist);
+  switch (pick_e(fwd.log2n)) {
+    case 2: return launch_key<2>(fwd, key, o, npolys, t, z, s);
+    case 4: return launch_key<4>(fwd, key, o, npolys, t, z, s);
+    case 8: return launch_key<8>(fwd, key, o, npolys, t, z, s);
+    default: return launch_key<16>(fwd, key, o, npolys, t, z, s);
+  }
+}
+
+hipError_t external_product(const Program &fwd, const Program &inv, const dc64 *keyf,
+                            const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
+                            uint32_t levels, uint32_t base_log, const dc64 *tw,
+                            const dc64 *twist, const dc64 *untwist, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const c64 *kf = reinterpret_cast<const c64 *>(keyf);
+  const c64 *t = reinterpret_cast<const c64 *>(tw), *z = reinterpret_cast<const c64 *>(twist);
+  const c64 *u = reinterpret_cast<const c64 *>(untwist);
+  switch (pick_e(fwd.log2n)) {
+    case 2: return launch_ext<2>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
+    case 4: return launch_ext<4>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
+    case 8: return launch_ext<8>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
+    default:
+      return launch_ext<16>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
+  }
+}
+
+}  // namespace cfft_launch

@@ -1,5 +1,6 @@
 // products_api.cpp -- C ABI of the fused products built on an unordered plan:
-// the exact limb-split u64 product.
+// the exact limb-split u64 product and the TFHE external product on a
+// resident Fourier-domain GGSW key.
 #include <hip/hip_runtime.h>
 
 #include "cfft.h"
@@ -14,7 +15,30 @@ hipError_t exact_polymul(const cfft_prog::Program &fwd, const cfft_prog::Program
                          const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
                          uint64_t batch, const dc64 *tw, const dc64 *twist, const dc64 *untwist,
                          dc64 *F, hipStream_t s);
+hipError_t torus_to_fourier(const cfft_prog::Program &fwd, const uint64_t *key, dc64 *out,
+                            uint64_t npolys, const dc64 *tw, const dc64 *twist, hipStream_t s);
+hipError_t external_product(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                            const dc64 *keyf, const uint64_t *glwe, uint64_t *out, uint64_t batch,
+                            uint32_t k1, uint32_t levels, uint32_t base_log, const dc64 *tw,
+                            const dc64 *twist, const dc64 *untwist, hipStream_t s);
+size_t external_product_lds_bytes(uint32_t log2n, uint32_t k1);
 }
+
+namespace {
+int check_extprod_plan(cfft_plan plan) {
+  if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (plan->kind != cfft_plan_s::UNORDERED)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "the external product runs on an unordered plan");
+  if (plan->n < 2 || plan->fwd_full.npass == 0)
+    return set_error(CFFT_ERROR_SIZE, "external product supports 4 <= N <= 2 * 8192");
+  return CFFT_SUCCESS;
+}
+uint32_t ilog2(size_t n) {
+  uint32_t l = 0;
+  while (((size_t)1 << l) < n) ++l;
+  return l;
+}
+}  // namespace
 
 extern "C" {
 
@@ -39,6 +63,47 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
   hipError_t e2 = hipFreeAsync(F, s);
   if (e != hipSuccess) return hip_error(e, "exact product");
   return hip_error(e2, "hipFreeAsync");
+}
+
+int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels, void *key_fourier,
+                         cfft_stream_t stream) {
+  int st = check_extprod_plan(plan);
+  if (st != CFFT_SUCCESS) return st;
+  if (k1 < 1 || levels < 1) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "k1 and levels must be >= 1");
+  if (!key || !key_fourier) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  const uint64_t npolys = (uint64_t)k1 * (uint64_t)levels * (uint64_t)k1;
+  return hip_error(cfft_launch::torus_to_fourier(plan->fwd_full, key,
+                                                 reinterpret_cast<cfft_launch::dc64 *>(key_fourier),
+                                                 npolys, plan->d_tw, plan->d_twist,
+                                                 (hipStream_t)stream),
+                   "ggsw to fourier");
+}
+
+int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int levels,
+                          int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
+                          cfft_stream_t stream) {
+  int st = check_extprod_plan(plan);
+  if (st != CFFT_SUCCESS) return st;
+  if (k1 < 1 || levels < 1) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "k1 and levels must be >= 1");
+  if (base_log < 1 || base_log > 32 || levels * base_log > 64)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT,
+                     "gadget base_log must be in [1, 32] with levels * base_log <= 64");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  int dev = 0, maxb = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return hip_error(e, "device query");
+  const size_t need = cfft_launch::external_product_lds_bytes(ilog2(plan->n), (uint32_t)k1);
+  if (need > (size_t)maxb)
+    return set_error(CFFT_ERROR_SIZE, "external product: k1 Fourier accumulators of this size do "
+                                      "not fit in one workgroup's LDS");
+  return hip_error(cfft_launch::external_product(
+                       plan->fwd_full, plan->inv_full,
+                       reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
+                       (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
+                       plan->d_twist, plan->d_untwist, (hipStream_t)stream),
+                   "external product");
 }
 
 }  // extern "C"

@@ -212,14 +212,20 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
 
 /* ---- external product (TFHE GGSW x GLWE; caller-side in the reference) ---- */
 /* Transform a GGSW key of k1*levels rows x k1 columns of u64 torus
- * polynomials (N each) into the Fourier domain of `plan` (device, in place
- * layout [row][col][n] c64). */
+ * polynomials (key: [row][col][N] u64, row = p * levels + i for GLWE poly p
+ * and gadget level i) into the Fourier domain of `plan` (key_fourier:
+ * [row][col][n] c64 in the plan's permuted order, the layout
+ * fourier_to_standard / serialize_fourier_buffer take).  Device buffers. */
 int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels, void *key_fourier,
                          cfft_stream_t stream);
-/* out (k1 polys) = GGSW(key_fourier) (x) GLWE(glwe): gadget decomposition
- * (levels x base 2^base_log, balanced digits), forward transforms, Fourier
- * MAC against the resident key, inverse, torus rounding.  Device, batch of
- * independent GLWEs sharing one key. */
+/* out (batch x k1 polys of N u64) = GGSW(key_fourier) (x) GLWE(glwe, batch x
+ * k1 polys): gadget decomposition (round to the nearest multiple of
+ * 2^(64 - levels*base_log), ties up; balanced digits in [-2^(B-1), 2^(B-1)),
+ * level 0 most significant), forward transforms, Fourier multiply-accumulate
+ * against the resident key (acc += key * digit, rows in order from acc = 0),
+ * inverse, untwist, torus rounding.  1 <= base_log <= 32,
+ * levels * base_log <= 64; the (1 + k1) Fourier buffers of one GLWE must fit
+ * in one workgroup's LDS (k1 <= 3 at N = 4096).  Device buffers. */
 int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int levels,
                           int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
                           cfft_stream_t stream);

@@ -213,6 +213,32 @@ void or_gadget_decompose(size_t N, const uint64_t *in, int levels, int base_log,
   }
 }
 
+/* Torus polynomials -> Fourier domain of the unordered plan: fold,
+ * (i64)x * 2^-64, twist, forward transform (the A-operand path of the
+ * product above).  in: npolys x N u64, out: npolys x N/2 c64. */
+int or_torus_to_fourier(size_t N, int base_algo, size_t base_n, const uint64_t *in, c64 *out,
+                        size_t npolys) {
+  size_t n = N / 2;
+  or_uplan plan;
+  int rc = or_uplan_new(&plan, n, base_algo, base_n);
+  if (rc) return rc;
+  c64 *tw = (c64 *)malloc(n * sizeof(c64));
+  c64 *scratch = (c64 *)malloc(base_n * sizeof(c64));
+  or_twist_table(N, tw);
+  for (size_t p = 0; p < npolys; p++) {
+    const uint64_t *a = in + p * N;
+    c64 *z = out + p * n;
+    for (size_t j = 0; j < n; j++) {
+      c64 x = {(double)(int64_t)a[j] * TWO_M64, (double)(int64_t)a[j + n] * TWO_M64};
+      z[j] = cmul(tw[j], x);
+    }
+    or_uplan_fwd(&plan, z, scratch);
+  }
+  free(tw); free(scratch);
+  or_uplan_free(&plan);
+  return 0;
+}
+
 /* External product (GGSW in the Fourier domain x GLWE):
  *   glwe: k1 polys of N u64 (k1 = k+1), decomposed into k1*levels digit
  *   polys (row = p*levels + i);  key_fourier: [row][col][n] c64 in the

@@ -41,6 +41,7 @@ class Oracle:
             "or_fft128_fwd": (None, [c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
             "or_fft128_inv": (None, [c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
             "or_gadget_decompose": (None, [c_size, c_vp, c_int, c_int, c_vp]),
+            "or_torus_to_fourier": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_size]),
             "or_external_product": (c_int, [c_size, c_int, c_int, c_int, c_vp, c_int, c_size, c_vp, c_vp]),
             "or_time_polymul": (ctypes.c_double, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
         }
@@ -156,6 +157,15 @@ class Oracle:
         x = np.ascontiguousarray(x, dtype=np.uint64)
         out = np.zeros((levels, x.size), dtype=np.int64)
         self.L.or_gadget_decompose(x.size, x.ctypes.data, levels, base_log, out.ctypes.data)
+        return out
+
+    def torus_to_fourier(self, polys, base_algo, base_n):
+        polys = np.ascontiguousarray(polys, dtype=np.uint64)
+        N = polys.shape[-1]
+        out = np.zeros(polys.shape[:-1] + (N // 2,), dtype=np.complex128)
+        rc = self.L.or_torus_to_fourier(N, int(base_algo), base_n, polys.ctypes.data,
+                                        out.ctypes.data, polys.size // N)
+        assert rc == 0
         return out
 
     def external_product(self, N, k1, levels, base_log, key_fourier, base_algo, base_n, glwe):

@@ -89,3 +89,55 @@ def test_gadget_decomposition(O, levels, base_log):
         rounded = ((v >> (64 - rep - 1)) + 1) >> 1 << (64 - rep) if rep < 64 else v
         recomposed = sum(int(d[i, j]) << (64 - base_log * (i + 1)) for i in range(levels))
         assert recomposed & M64 == rounded & M64, (j, v)
+
+
+def gadget_key(N, k1, levels, base_log):
+    """Trivial GGSW of the message 1: row (p, i), column c holds the constant
+    polynomial delta_{p,c} * 2^(64 - (i+1) B)."""
+    key = np.zeros((k1 * levels, k1, N), dtype=np.uint64)
+    for p in range(k1):
+        for i in range(levels):
+            key[p * levels + i, p, 0] = np.uint64(1 << (64 - (i + 1) * base_log))
+    return key
+
+
+def test_torus_to_fourier_constant_polynomial(O):
+    # a constant polynomial c has the flat spectrum c * 2^-64 in every slot
+    N = 512
+    for c in (1 << 63, 3 << 60, (1 << 64) - (1 << 40)):
+        poly = np.zeros((1, N), dtype=np.uint64)
+        poly[0, 0] = np.uint64(c)
+        f = O.torus_to_fourier(poly, 6, 256)[0]
+        want = float(np.int64(np.uint64(c).view(np.int64))) * 2.0 ** -64
+        assert np.all(f.real == want) and np.all(f.imag == 0.0)
+
+
+@pytest.mark.parametrize("N,k1,levels,base_log", [(256, 2, 3, 8), (1024, 1, 2, 10), (64, 3, 4, 6)])
+def test_external_product_trivial_ggsw_recomposes(O, N, k1, levels, base_log):
+    # GGSW(1) (x) GLWE = gadget recomposition of the rounded GLWE
+    rng = np.random.default_rng(N + k1)
+    glwe = rng.integers(0, 2**64, (k1, N), dtype=np.uint64)
+    kf = O.torus_to_fourier(gadget_key(N, k1, levels, base_log), 6, min(N // 2, 256))
+    out = O.external_product(N, k1, levels, base_log, kf, 6, min(N // 2, 256), glwe)
+    rep = levels * base_log
+    half = np.uint64(1 << (64 - rep - 1))
+    rounded = ((glwe + half) >> np.uint64(64 - rep)) << np.uint64(64 - rep)
+    err = np.abs((out - rounded).view(np.int64)).max()
+    assert err < 2**16, err
+
+
+def test_external_product_coarse_grid_matches_schoolbook(O):
+    # key on the 2^44 grid: the f64 product is exact after rounding to it
+    N, k1, levels, base_log = 256, 2, 3, 8
+    rng = np.random.default_rng(11)
+    key = (rng.integers(-2**12, 2**12, (k1 * levels, k1, N), dtype=np.int64) << 44).view(np.uint64)
+    glwe = rng.integers(0, 2**64, (k1, N), dtype=np.uint64)
+    kf = O.torus_to_fourier(key, 2, 32)
+    out = O.external_product(N, k1, levels, base_log, kf, 2, 32, glwe)
+    digits = np.concatenate([O.gadget_decompose(glwe[p], levels, base_log) for p in range(k1)])
+    want = np.zeros((k1, N), dtype=np.uint64)
+    for c in range(k1):
+        for r in range(k1 * levels):
+            want[c] += O.schoolbook_u64(key[r, c][None], digits[r].view(np.uint64)[None])[0]
+    got = ((out + np.uint64(1 << 43)) >> np.uint64(44)) << np.uint64(44)
+    assert np.array_equal(got, want)
