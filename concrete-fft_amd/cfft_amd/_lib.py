@@ -87,6 +87,15 @@ _SIGS = [
     ("cfft_exact_polymul_u64", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     ("cfft_ggsw_to_fourier", c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     ("cfft_external_product", c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_size, c_vp]),
+    ("cfft_uneven_create", c_int, [c_vp, c_size, c_size, c_size, c_size, c_int, c_int, c_dbl,
+                                   ctypes.POINTER(c_vp)]),
+    ("cfft_uneven_destroy", c_int, [c_vp]),
+    ("cfft_uneven_run", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("cfft_uneven_run_plain", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("cfft_uneven_class_algo", c_int, [c_vp, ctypes.c_uint32, ctypes.POINTER(c_int),
+                                       ctypes.POINTER(c_size)]),
+    ("cfft_uneven_info", c_int, [c_vp, ctypes.POINTER(c_size), ctypes.POINTER(c_size),
+                                 ctypes.POINTER(c_size), ctypes.POINTER(ctypes.c_uint32)]),
 ]
 
 for _name, _res, _args in _SIGS:

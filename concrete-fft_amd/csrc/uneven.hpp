@@ -1,0 +1,58 @@
+// uneven.hpp -- device-side records of an uneven-batch schedule (shared by
+// the host runtime in uneven_api.cpp and the kernels in uneven_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "launch.hpp"
+#include "program.hpp"
+
+namespace cfft_ub {
+
+constexpr uint32_t kThreads = 512;  // one workgroup: 8 waves
+// LDS of one workgroup: (8192 / n) transform groups of n + n/16 points
+constexpr uint32_t kLdsPoints = 8192 + 512;
+constexpr size_t kLdsBytes = kLdsPoints * 16;
+constexpr uint32_t kMinLog2N = 5;   // N = 32: n = 16, one thread per transform
+constexpr uint32_t kMaxLog2N = 14;  // N = 16384: n = 8192, one transform per workgroup
+
+struct UJob {
+  uint64_t in_off;   // doubles
+  uint64_t f_off;    // c64
+  uint64_t out_off;  // doubles
+  uint32_t log2n;    // complex length n = N / 2
+  uint32_t pad;
+};
+
+// `count` jobs order[first .. first + count) of class `cls`, one per group
+struct UTile {
+  uint32_t cls;
+  uint32_t first;
+  uint32_t count;
+  uint32_t pad;
+};
+
+struct UClass {
+  cfft_prog::Program fwd, inv;  // whole-transform programs of the class plan
+  const cfft_launch::dc64 *tw;  // T[j], n entries
+  const cfft_launch::dc64 *twist;
+  const cfft_launch::dc64 *untwist;
+  uint32_t log2n;
+  uint32_t pad;
+};
+
+// groups (jobs) per tile for a class of complex length 2^log2n
+constexpr uint32_t groups_per_tile(uint32_t log2n) { return kThreads / ((1u << log2n) / 16); }
+
+// sets the kernels' LDS limit on the current device (before any launch)
+hipError_t prepare();
+hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+                             uint32_t ntiles, const UClass *classes, uint32_t *counter,
+                             uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
+                             double *out, hipStream_t s);
+hipError_t launch_tiles(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+                        uint32_t first_tile, uint32_t count, const UClass *classes,
+                        const double *in, cfft_launch::dc64 *fourier, double *out, hipStream_t s);
+
+}  // namespace cfft_ub

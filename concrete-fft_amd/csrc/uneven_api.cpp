@@ -1,0 +1,399 @@
+// uneven_api.cpp -- C ABI of the uneven batch: many f64 negacyclic round-trip
+// jobs of mixed N, run as one persistent launch fed by a device tile queue
+// (replayed from a captured HIP graph), or in the plain form of one launch
+// per size class.
+//
+// A schedule is built once from the job list: one unordered plan per size
+// class (fixed: the caller's algorithm with base_n = min(n, 1024); measured:
+// the fastest of a candidate set, timed on the device with this schedule's
+// own kernels), tiles of up to 8192/n jobs of one class, queued largest class
+// first so the long tiles start early and the short ones fill the tail.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "cfft.h"
+#include "plan.hpp"
+#include "uneven.hpp"
+
+using cfft_internal::hip_error;
+using cfft_internal::set_error;
+using cfft_launch::dc64;
+using cfft_ub::UClass;
+using cfft_ub::UJob;
+using cfft_ub::UTile;
+
+namespace {
+
+struct DevSchedule {
+  UJob *jobs = nullptr;
+  uint32_t *order = nullptr;
+  UTile *tiles = nullptr;
+  UClass *classes = nullptr;
+  uint32_t *counter = nullptr;
+  uint32_t ntiles = 0;
+  void release() {
+    if (jobs) (void)hipFree(jobs);
+    if (order) (void)hipFree(order);
+    if (tiles) (void)hipFree(tiles);
+    if (classes) (void)hipFree(classes);
+    if (counter) (void)hipFree(counter);
+    *this = DevSchedule{};
+  }
+};
+
+struct ClassRange {
+  uint32_t log2n = 0;
+  uint32_t first_tile = 0, ntiles = 0;
+};
+
+// tiles of `order` (job indices grouped by class, classes in `class_seq`)
+struct HostSchedule {
+  std::vector<uint32_t> order;
+  std::vector<UTile> tiles;
+  std::vector<ClassRange> ranges;  // per class, in queue order
+};
+
+template <class T>
+hipError_t upload(T **dst, const std::vector<T> &src) {
+  if (src.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), src.size() * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// jobs_by_class[c]: job indices of class c (complex length 2^log2n[c]);
+// classes are queued in descending size
+HostSchedule make_schedule(const std::vector<std::vector<uint32_t>> &jobs_by_class,
+                           const std::vector<uint32_t> &class_log2n) {
+  HostSchedule h;
+  std::vector<uint32_t> seq(jobs_by_class.size());
+  for (uint32_t c = 0; c < seq.size(); ++c) seq[c] = c;
+  std::stable_sort(seq.begin(), seq.end(),
+                   [&](uint32_t a, uint32_t b) { return class_log2n[a] > class_log2n[b]; });
+  for (uint32_t c : seq) {
+    const auto &js = jobs_by_class[c];
+    if (js.empty()) continue;
+    ClassRange r;
+    r.log2n = class_log2n[c];
+    r.first_tile = (uint32_t)h.tiles.size();
+    const uint32_t G = cfft_ub::groups_per_tile(class_log2n[c]);
+    const uint32_t base = (uint32_t)h.order.size();
+    h.order.insert(h.order.end(), js.begin(), js.end());
+    for (uint32_t i = 0; i < js.size(); i += G) {
+      UTile t;
+      t.cls = c;
+      t.first = base + i;
+      t.count = std::min<uint32_t>(G, (uint32_t)js.size() - i);
+      t.pad = 0;
+      h.tiles.push_back(t);
+    }
+    r.ntiles = (uint32_t)h.tiles.size() - r.first_tile;
+    h.ranges.push_back(r);
+  }
+  return h;
+}
+
+hipError_t upload_schedule(DevSchedule &d, const std::vector<UJob> &jobs, const HostSchedule &h,
+                           const std::vector<UClass> &classes) {
+  hipError_t e = upload(&d.jobs, jobs);
+  if (e == hipSuccess) e = upload(&d.order, h.order);
+  if (e == hipSuccess) e = upload(&d.tiles, h.tiles);
+  if (e == hipSuccess) e = upload(&d.classes, classes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.counter), 256);
+  d.ntiles = (uint32_t)h.tiles.size();
+  return e;
+}
+
+UClass class_of(const cfft_plan_s *p) {
+  UClass c{};
+  c.fwd = p->fwd_full;
+  c.inv = p->inv_full;
+  c.tw = p->d_tw;
+  c.twist = p->d_twist;
+  c.untwist = p->d_untwist;
+  uint32_t l = 0;
+  while (((size_t)1 << l) < p->n) ++l;
+  c.log2n = l;
+  return c;
+}
+
+struct Candidate {
+  int algo;
+  size_t base_n;
+};
+
+std::vector<Candidate> candidates(uint32_t log2n) {
+  const size_t n = (size_t)1 << log2n;
+  std::vector<Candidate> out;
+  std::vector<size_t> bases{std::min<size_t>(n, 1024)};
+  if (n >= 1024) bases.push_back(512);
+  for (size_t b : bases)
+    for (int a : {CFFT_DIF4, CFFT_DIF8, CFFT_DIF16, CFFT_DIT16}) out.push_back({a, b});
+  return out;
+}
+
+int grid_size(uint32_t *grid) {
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return hip_error(e, "device query");
+  if (cus <= 0) return set_error(CFFT_ERROR_DEVICE, "device reports no compute units");
+  int maxlds = 0;
+  e = hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return hip_error(e, "device query");
+  if ((size_t)maxlds < cfft_ub::kLdsBytes + 64)
+    return set_error(CFFT_ERROR_SIZE, "device LDS per workgroup is below the uneven kernel's need");
+  *grid = (uint32_t)cus;  // the workgroup's LDS fills a CU
+  return CFFT_SUCCESS;
+}
+
+}  // namespace
+
+struct cfft_uneven_s {
+  int device = 0;
+  size_t in_len = 0, f_len = 0, out_len = 0;
+  std::vector<UJob> jobs;
+  std::vector<uint32_t> class_log2n;       // per class
+  std::vector<cfft_plan> plans;            // per class
+  HostSchedule host;
+  DevSchedule dev;
+  uint32_t grid = 0;
+  std::mutex mu;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *g_in = nullptr, *g_f = nullptr, *g_out = nullptr;
+  ~cfft_uneven_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+    dev.release();
+    for (cfft_plan p : plans)
+      if (p) cfft_plan_destroy(p);
+  }
+};
+
+namespace {
+
+// time one class alone with a candidate plan on packed synthetic buffers
+int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t grid, double *ms) {
+  cfft_plan p = nullptr;
+  int rc = cfft_unordered_plan_create((size_t)1 << log2n, CFFT_METHOD_USER_PROVIDED, c.algo,
+                                      c.base_n, 0.0, &p);
+  if (rc) return rc;
+  const size_t n = (size_t)1 << log2n;
+  std::vector<UJob> jobs(njobs);
+  for (size_t i = 0; i < njobs; ++i) jobs[i] = UJob{2 * n * i, n * i, 2 * n * i, (uint32_t)log2n, 0};
+  std::vector<std::vector<uint32_t>> by_class(1);
+  for (uint32_t i = 0; i < njobs; ++i) by_class[0].push_back(i);
+  HostSchedule h = make_schedule(by_class, {log2n});
+  std::vector<UClass> cls{class_of(p)};
+  DevSchedule d;
+  double *x = nullptr, *y = nullptr;
+  dc64 *f = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = upload_schedule(d, jobs, h, cls);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x), 2 * n * njobs * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&y), 2 * n * njobs * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&f), n * njobs * sizeof(dc64));
+  if (e == hipSuccess) e = hipMemset(x, 0, 2 * n * njobs * sizeof(double));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double best = 1e30;
+  for (int it = 0; it < 4 && e == hipSuccess; ++it) {
+    e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess)
+      e = cfft_ub::launch_persistent(d.jobs, d.order, d.tiles, d.ntiles, d.classes, d.counter,
+                                         grid, x, f, y, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess && it > 0) best = std::min(best, (double)t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (x) (void)hipFree(x);
+  if (y) (void)hipFree(y);
+  if (f) (void)hipFree(f);
+  d.release();
+  cfft_plan_destroy(p);
+  if (e != hipSuccess) return hip_error(e, "uneven autotuner");
+  *ms = best;
+  return CFFT_SUCCESS;
+}
+
+int check_job(const cfft_uneven_job &j, size_t idx, size_t in_len, size_t f_len, size_t out_len) {
+  char m[200];
+  if (j.log2_N < cfft_ub::kMinLog2N || j.log2_N > cfft_ub::kMaxLog2N) {
+    snprintf(m, sizeof m, "job %zu: log2_N = %u outside [%u, %u]", idx, j.log2_N,
+             cfft_ub::kMinLog2N, cfft_ub::kMaxLog2N);
+    return set_error(CFFT_ERROR_SIZE, m);
+  }
+  const uint64_t N = (uint64_t)1 << j.log2_N;
+  if (j.in_offset > in_len || in_len - j.in_offset < N || j.out_offset > out_len ||
+      out_len - j.out_offset < N || j.fourier_offset > f_len || f_len - j.fourier_offset < N / 2) {
+    snprintf(m, sizeof m, "job %zu: buffer range outside the stated buffer lengths", idx);
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, m);
+  }
+  return CFFT_SUCCESS;
+}
+
+int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int method, int base_algo,
+          double seconds) {
+  (void)seconds;
+  std::vector<std::vector<uint32_t>> by_log2n(cfft_ub::kMaxLog2N + 1);
+  u->jobs.resize(njobs);
+  for (size_t i = 0; i < njobs; ++i) {
+    const uint32_t l = jobs[i].log2_N - 1;
+    u->jobs[i] = UJob{jobs[i].in_offset, jobs[i].fourier_offset, jobs[i].out_offset, l, 0};
+    by_log2n[l].push_back((uint32_t)i);
+  }
+  int rc = grid_size(&u->grid);
+  if (rc) return rc;
+  hipError_t ep = cfft_ub::prepare();
+  if (ep != hipSuccess) return hip_error(ep, "uneven kernel attributes");
+  std::vector<std::vector<uint32_t>> by_class;
+  std::vector<UClass> classes;
+  for (uint32_t l = 0; l < by_log2n.size(); ++l) {
+    if (by_log2n[l].empty()) continue;
+    const size_t n = (size_t)1 << l;
+    Candidate pick{base_algo, std::min<size_t>(n, 1024)};
+    if (method == CFFT_METHOD_MEASURE) {
+      // time on up to 4 tiles per workgroup of this class
+      const size_t sample = std::min<size_t>(
+          by_log2n[l].size(), (size_t)4 * u->grid * cfft_ub::groups_per_tile(l));
+      double best = 1e30;
+      for (const Candidate &c : candidates(l)) {
+        double ms = 0;
+        rc = time_candidate(l, sample, c, u->grid, &ms);
+        if (rc) return rc;
+        if (ms < best) {
+          best = ms;
+          pick = c;
+        }
+      }
+    }
+    cfft_plan p = nullptr;
+    rc = cfft_unordered_plan_create(n, CFFT_METHOD_USER_PROVIDED, pick.algo, pick.base_n, 0.0, &p);
+    if (rc) return rc;
+    u->plans.push_back(p);
+    u->class_log2n.push_back(l);
+    classes.push_back(class_of(p));
+    by_class.push_back(by_log2n[l]);
+  }
+  u->host = make_schedule(by_class, u->class_log2n);
+  hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  return hip_error(e, "uneven schedule upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+int cfft_uneven_create(const cfft_uneven_job *jobs, size_t njobs, size_t in_len,
+                       size_t fourier_len, size_t out_len, int method, int base_algo,
+                       double measure_seconds, cfft_uneven *out) {
+  if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");
+  *out = nullptr;
+  if (njobs && !jobs) return set_error(CFFT_ERROR_NULL_POINTER, "jobs is null");
+  if (njobs > 0xffffffffull) return set_error(CFFT_ERROR_SIZE, "too many jobs for one schedule");
+  if (method != CFFT_METHOD_USER_PROVIDED && method != CFFT_METHOD_MEASURE)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown method");
+  if (method == CFFT_METHOD_USER_PROVIDED && (base_algo < CFFT_DIF2 || base_algo > CFFT_DIT16))
+    return set_error(CFFT_ERROR_INVALID_ALGO, "unknown FftAlgo");
+  for (size_t i = 0; i < njobs; ++i) {
+    int rc = check_job(jobs[i], i, in_len, fourier_len, out_len);
+    if (rc) return rc;
+  }
+  cfft_uneven_s *u = new cfft_uneven_s;
+  u->in_len = in_len;
+  u->f_len = fourier_len;
+  u->out_len = out_len;
+  (void)hipGetDevice(&u->device);
+  int rc = build(u, jobs, njobs, method, base_algo, measure_seconds);
+  if (rc) {
+    delete u;
+    return rc;
+  }
+  *out = u;
+  return CFFT_SUCCESS;
+}
+
+int cfft_uneven_destroy(cfft_uneven u) {
+  delete u;
+  return CFFT_SUCCESS;
+}
+
+int cfft_uneven_class_algo(cfft_uneven u, uint32_t log2_N, int *algo, size_t *base_n) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  for (size_t c = 0; c < u->class_log2n.size(); ++c) {
+    if (u->class_log2n[c] + 1 != log2_N) continue;
+    if (algo) *algo = u->plans[c]->algo;
+    if (base_n) *base_n = u->plans[c]->base_n;
+    return CFFT_SUCCESS;
+  }
+  return set_error(CFFT_ERROR_INVALID_ARGUMENT, "no job of this size in the schedule");
+}
+
+int cfft_uneven_info(cfft_uneven u, size_t *njobs, size_t *ntiles, size_t *nclasses,
+                     uint32_t *grid) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (njobs) *njobs = u->jobs.size();
+  if (ntiles) *ntiles = u->host.tiles.size();
+  if (nclasses) *nclasses = u->class_log2n.size();
+  if (grid) *grid = u->grid;
+  return CFFT_SUCCESS;
+}
+
+int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *out,
+                    cfft_stream_t stream) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (u->jobs.empty()) return CFFT_SUCCESS;
+  if (!in || !fourier || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  std::lock_guard<std::mutex> g(u->mu);
+  hipError_t e = hipSuccess;
+  if (!u->exec || u->g_in != in || u->g_f != fourier || u->g_out != out) {
+    if (u->exec) (void)hipGraphExecDestroy(u->exec);
+    if (u->graph) (void)hipGraphDestroy(u->graph);
+    u->exec = nullptr;
+    u->graph = nullptr;
+    e = hipStreamBeginCapture(u->cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t el = cfft_ub::launch_persistent(
+          u->dev.jobs, u->dev.order, u->dev.tiles, u->dev.ntiles, u->dev.classes, u->dev.counter,
+          u->grid, in, reinterpret_cast<dc64 *>(fourier), out, u->cap);
+      e = hipStreamEndCapture(u->cap, &u->graph);
+      if (el != hipSuccess) e = el;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&u->exec, u->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return hip_error(e, "uneven graph capture");
+    u->g_in = in;
+    u->g_f = fourier;
+    u->g_out = out;
+  }
+  return hip_error(hipGraphLaunch(u->exec, (hipStream_t)stream), "uneven graph launch");
+}
+
+int cfft_uneven_run_plain(cfft_uneven u, const double *in, double *fourier, double *out,
+                          cfft_stream_t stream) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (u->jobs.empty()) return CFFT_SUCCESS;
+  if (!in || !fourier || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  for (const ClassRange &r : u->host.ranges) {
+    hipError_t e = cfft_ub::launch_tiles(u->dev.jobs, u->dev.order, u->dev.tiles,
+                                             r.first_tile, r.ntiles, u->dev.classes, in,
+                                             reinterpret_cast<dc64 *>(fourier), out,
+                                             (hipStream_t)stream);
+    if (e != hipSuccess) return hip_error(e, "uneven plain launch");
+  }
+  return CFFT_SUCCESS;
+}
+
+}  // extern "C"

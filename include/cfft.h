@@ -230,6 +230,44 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
                           int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
                           cfft_stream_t stream);
 
+/* ---- uneven batch (no counterpart in the reference, which runs one plan
+ *      per call): f64 negacyclic round trips of mixed N in one launch ---- */
+/* A job: N = 2^log2_N real coefficients at in + in_offset (doubles) are
+ * folded, twisted and forward transformed with its size class's unordered
+ * plan into fourier + fourier_offset (N/2 c64, the plan's order); the inverse
+ * transform and untwist (with 1/n) write N doubles at out + out_offset.
+ * 5 <= log2_N <= 14. */
+typedef struct {
+  uint64_t in_offset;
+  uint64_t fourier_offset;
+  uint64_t out_offset;
+  uint32_t log2_N;
+  uint32_t reserved;
+} cfft_uneven_job;
+typedef struct cfft_uneven_s *cfft_uneven;
+/* Builds a schedule for a job list (host array) over buffers of the stated
+ * lengths (in and out in doubles, fourier in c64; every job is checked to lie
+ * inside them).  method USER_PROVIDED: every size class uses base_algo with
+ * base_n = min(n, 1024); MEASURE: each class uses the fastest of a candidate
+ * set of plans, timed on the device (the choice replaces Method::Measure,
+ * src/unordered.rs:553-640, for this workload).  Blocks until built. */
+int cfft_uneven_create(const cfft_uneven_job *jobs, size_t njobs, size_t in_len,
+                       size_t fourier_len, size_t out_len, int method, int base_algo,
+                       double measure_seconds, cfft_uneven *out);
+int cfft_uneven_destroy(cfft_uneven u);
+/* One persistent launch over a device tile queue, replayed from a HIP graph
+ * captured for these buffer addresses (re-captured when they change).  Runs
+ * of one schedule must not overlap (they share its queue counter). */
+int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *out,
+                    cfft_stream_t stream);
+/* The plain form: one launch per size class, one workgroup per tile. */
+int cfft_uneven_run_plain(cfft_uneven u, const double *in, double *fourier, double *out,
+                          cfft_stream_t stream);
+/* The plan a size class runs with (which fixes its Fourier order). */
+int cfft_uneven_class_algo(cfft_uneven u, uint32_t log2_N, int *algo, size_t *base_n);
+int cfft_uneven_info(cfft_uneven u, size_t *njobs, size_t *ntiles, size_t *nclasses,
+                     uint32_t *grid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,8 @@ void or_fft128_inv(size_t n, double *re0, double *re1, double *im0, double *im1,
 
 /* ---- external product (TFHE GGSW x GLWE, caller-side; see polymul.c) ---- */
 void or_gadget_decompose(size_t N, const uint64_t *in, int levels, int base_log, int64_t *out);
+int or_negacyclic_roundtrip_f64(size_t N, int base_algo, size_t base_n, const double *x,
+                                or_c64 *F, double *out, size_t count);
 int or_torus_to_fourier(size_t N, int base_algo, size_t base_n, const uint64_t *in,
                         or_c64 *out, size_t npolys);
 int or_external_product(size_t N, int k1, int levels, int base_log, const or_c64 *key_fourier,

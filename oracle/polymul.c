@@ -213,6 +213,43 @@ void or_gadget_decompose(size_t N, const uint64_t *in, int levels, int base_log,
   }
 }
 
+/* f64 negacyclic round trip of real polynomials (the uneven-batch job):
+ * fold + twist + forward transform -> F (n c64, plan order), then inverse +
+ * untwist (with 1/n) -> out (N f64).  x, out: count x N; F: count x n. */
+int or_negacyclic_roundtrip_f64(size_t N, int base_algo, size_t base_n, const double *x,
+                                c64 *F, double *out, size_t count) {
+  if (N < 2 || (N & (N - 1))) return -1;
+  size_t n = N / 2;
+  or_uplan plan;
+  int rc = or_uplan_new(&plan, n, base_algo, base_n);
+  if (rc) return rc;
+  c64 *tw = (c64 *)malloc(n * sizeof(c64));
+  c64 *u = (c64 *)malloc(n * sizeof(c64));
+  c64 *z = (c64 *)malloc(n * sizeof(c64));
+  c64 *scratch = (c64 *)malloc(base_n * sizeof(c64));
+  or_twist_table(N, tw);
+  make_untwist(N, tw, u);
+  for (size_t p = 0; p < count; p++) {
+    const double *xp = x + p * N;
+    for (size_t j = 0; j < n; j++) {
+      c64 v = {xp[j], xp[j + n]};
+      z[j] = cmul(tw[j], v);
+    }
+    or_uplan_fwd(&plan, z, scratch);
+    for (size_t k = 0; k < n; k++) F[p * n + k] = z[k];
+    or_uplan_inv(&plan, z, scratch);
+    double *o = out + p * N;
+    for (size_t j = 0; j < n; j++) {
+      c64 v = cmul(u[j], z[j]);
+      o[j] = v.re;
+      o[j + n] = v.im;
+    }
+  }
+  free(tw); free(u); free(z); free(scratch);
+  or_uplan_free(&plan);
+  return 0;
+}
+
 /* Torus polynomials -> Fourier domain of the unordered plan: fold,
  * (i64)x * 2^-64, twist, forward transform (the A-operand path of the
  * product above).  in: npolys x N u64, out: npolys x N/2 c64. */

@@ -41,6 +41,7 @@ class Oracle:
             "or_fft128_fwd": (None, [c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
             "or_fft128_inv": (None, [c_size, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_int]),
             "or_gadget_decompose": (None, [c_size, c_vp, c_int, c_int, c_vp]),
+            "or_negacyclic_roundtrip_f64": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size]),
             "or_torus_to_fourier": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_size]),
             "or_external_product": (c_int, [c_size, c_int, c_int, c_int, c_vp, c_int, c_size, c_vp, c_vp]),
             "or_time_polymul": (ctypes.c_double, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
@@ -158,6 +159,17 @@ class Oracle:
         out = np.zeros((levels, x.size), dtype=np.int64)
         self.L.or_gadget_decompose(x.size, x.ctypes.data, levels, base_log, out.ctypes.data)
         return out
+
+    def negacyclic_roundtrip(self, x, base_algo, base_n):
+        """(count, N) f64 -> (Fourier (count, N/2) c128, round trip (count, N) f64)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        N = x.shape[-1]
+        F = np.zeros(x.shape[:-1] + (N // 2,), dtype=np.complex128)
+        out = np.zeros_like(x)
+        rc = self.L.or_negacyclic_roundtrip_f64(N, int(base_algo), base_n, x.ctypes.data,
+                                                F.ctypes.data, out.ctypes.data, x.size // N)
+        assert rc == 0
+        return F, out
 
     def torus_to_fourier(self, polys, base_algo, base_n):
         polys = np.ascontiguousarray(polys, dtype=np.uint64)

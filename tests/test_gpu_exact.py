@@ -10,7 +10,7 @@ M = 2**64
 
 
 @pytest.mark.parametrize("N,algo,base_n", [(4, 6, 2), (64, 6, 32), (256, 2, 128),
-                                           (2048, 6, 1024), (2048, 3, 32), (4096, 6, 2048),
+                                           (2048, 6, 1024), (2048, 3, 32), (4096, 6, 1024),
                                            (4096, 0, 512)])
 def test_exact_bit_exact_random(cfft, oracle, N, algo, base_n):
     from cfft_amd import products

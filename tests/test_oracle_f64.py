@@ -140,3 +140,18 @@ def test_permutation_closed_form(O):
                 assert O.L.or_bit_rev_twice_inv(nb, bb, perm[i]) == i
             if bb == nb:
                 assert perm == list(range(1 << nb))
+
+
+@pytest.mark.parametrize("N,algo", [(64, 2), (512, 6), (2048, 4)])
+def test_negacyclic_roundtrip_closed_form(oracle, N, algo):
+    # base_n == n: the unordered order is the standard order, so F is the
+    # negacyclic evaluation sum_j (x_j + i x_{j+n}) e^{i pi j / N} e^{-2 pi i j k / n}
+    n = N // 2
+    rng = np.random.default_rng(N)
+    x = rng.uniform(-1, 1, (3, N))
+    F, y = oracle.negacyclic_roundtrip(x, algo, n)
+    j = np.arange(n)
+    z = (x[:, :n] + 1j * x[:, n:]) * np.exp(1j * np.pi * j / N)
+    want = z @ np.exp(-2j * np.pi * np.outer(j, j) / n)
+    assert np.abs(F - want).max() < 1e-12 * np.sqrt(n)
+    assert np.abs(y - x).max() < 1e-14 * np.log2(N)
