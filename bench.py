@@ -1,6 +1,6 @@
 """Benchmark: batched negacyclic polynomial products at N = 2048 on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode all|polymul|uneven|copy]
 
 Workload (the headline configuration): C = A * B mod (X^2048 + 1) on the
 2^-64 torus for a batch of 8192 independent products; A u64 torus
@@ -14,6 +14,14 @@ C written) per launch over its average launch time measured with HIP events
 on its stream; plain = the same products issued one call per polynomial;
 cpu_baseline = the oracle's restatement of the reference f64 path (one plan
 per thread over the host cores) on a bounded sample.
+
+Beside the headline, `uneven` reports the uneven batch: 20 000 f64
+negacyclic round-trip jobs (fold, twist, forward, inverse, untwist) with N
+drawn from {512, ..., 16384} with weights {.10, .20, .35, .20, .10, .05},
+values uniform in [-1, 1), as one persistent launch replayed from a HIP graph
+with autotuned per-size plans, against the fixed-schedule plain form (Dif4,
+one launch per size class).  `--mode copy` runs one device-to-device copy of
+--copy-mib MiB (the reference kernel for scaling HBM byte counters).
 """
 import argparse
 import json
@@ -31,6 +39,11 @@ N = 2048
 BATCH = 8192
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
+# HBM bytes per 8192-product launch of the persistent Dif16 kernel from
+# rocprofv3 --pmc: FETCH_SIZE x 2 (the counter reads half of a plain copy
+# kernel's known read size on this platform) + WRITE_SIZE (exact for a copy):
+# (131431 x 2 + 131072) KiB.
+TRAFFIC_BYTES = (131431 * 2 + 131072) * 1024
 
 
 def host_cores():
@@ -47,7 +60,7 @@ def make_inputs(seed=2):
     return a, b
 
 
-def cpu_baseline(a, b, algo, base_n, target_s=5.0):
+def cpu_baseline(a, b, algo, base_n, target_s=10.0):
     """Oracle restatement of the reference f64 path over all host cores: the
     whole batch once (its result is compared with the GPU's), then repeated
     passes until about target_s seconds of wall time."""
@@ -59,7 +72,7 @@ def cpu_baseline(a, b, algo, base_n, target_s=5.0):
                                   BATCH, threads)
     done = BATCH
     scratch = np.zeros_like(c)
-    while total_t < target_s and done < 64 * BATCH:
+    while total_t < target_s and done < 1024 * BATCH:
         total_t += O.L.or_time_polymul(N, algo, base_n, a.ctypes.data, b.ctypes.data,
                                        scratch.ctypes.data, BATCH, threads)
         done += BATCH
@@ -67,6 +80,86 @@ def cpu_baseline(a, b, algo, base_n, target_s=5.0):
             "sample": "%d products (%d passes over the N=2048 batch), %.1f s wall; oracle C "
                       "restatement of unordered::Plan (scalar, bit-identical to the reference "
                       "f64 path), one plan per thread" % (done, done // BATCH, total_t)}, c, BATCH
+
+
+UNEVEN_JOBS = 20000
+UNEVEN_LOG2N = [9, 10, 11, 12, 13, 14]
+UNEVEN_WEIGHTS = [0.10, 0.20, 0.35, 0.20, 0.10, 0.05]
+
+
+def time_runs(fn, stream, steps, warmup):
+    from cfft_amd.device import Event
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    evs = [(Event(), Event()) for _ in range(steps)]
+    for i in range(steps):
+        evs[i][0].record(stream)
+        fn()
+        evs[i][1].record(stream)
+    stream.synchronize()
+    return [e0.elapsed_ms(e1) for e0, e1 in evs]
+
+
+def bench_uneven(steps, warmup):
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.uneven import Schedule, packed_jobs
+    from cfft_amd.unordered import FftAlgo
+    rng = np.random.default_rng(3)
+    log2_N = rng.choice(UNEVEN_LOG2N, size=UNEVEN_JOBS, p=UNEVEN_WEIGHTS)
+    jobs, nin, nf, nout = packed_jobs(log2_N)
+    x = rng.uniform(-1.0, 1.0, nin)
+    s = Stream()
+    dx = DeviceBuffer.from_host(x, s)
+    dF = DeviceBuffer((nf,), np.complex128)
+    dy = DeviceBuffer((nout,), np.float64)
+    s.synchronize()
+    t0 = time.perf_counter()
+    tuned = Schedule(jobs, nin, nf, nout, measure=True)
+    tune_s = time.perf_counter() - t0
+    fixed = Schedule(jobs, nin, nf, nout, measure=False, algo=FftAlgo.Dif4)
+    forms = {
+        "tuned_persistent_graph": lambda: tuned.run(dx, dF, dy, s),
+        "tuned_plain": lambda: tuned.run_plain(dx, dF, dy, s),
+        "fixed_persistent_graph": lambda: fixed.run(dx, dF, dy, s),
+        "fixed_plain": lambda: fixed.run_plain(dx, dF, dy, s),
+    }
+    ms = {k: float(np.median(time_runs(f, s, steps, warmup))) for k, f in forms.items()}
+    tuned.run(dx, dF, dy, s)
+    s.synchronize()
+    err = float(np.abs(dy.download() - x).max())
+    total_N = int(np.sum(np.left_shift(1, log2_N.astype(np.int64))))
+    bytes_algo = 24 * total_N  # x read, Fourier written, round trip written (8N each)
+    best = ms["tuned_persistent_graph"]
+    plans = {}
+    for l in UNEVEN_LOG2N:
+        a, b = tuned.class_algo(l)
+        plans["N=%d" % (1 << l)] = "%s/%d" % (a.name, b)
+    return {
+        "workload": "%d f64 negacyclic fwd+inv jobs, N in {512..16384} weights %s, %d total "
+                    "coefficients" % (UNEVEN_JOBS, UNEVEN_WEIGHTS, total_N),
+        "jobs_per_s": round(UNEVEN_JOBS / (best * 1e-3), 1),
+        "transforms_per_s": round(2 * UNEVEN_JOBS / (best * 1e-3), 1),
+        "ms": {k: round(v, 4) for k, v in ms.items()},
+        "tuned_graph_over_fixed_plain": round(ms["fixed_plain"] / best, 3),
+        "roofline": {"bound": "hbm", "achieved": round(bytes_algo / (best * 1e-3) / 1e9, 2),
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(bytes_algo / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                     "bytes_per_launch": bytes_algo},
+        "tuned_plans": plans, "autotune_s": round(tune_s, 3),
+        "roundtrip_error_max": err, "schedule": tuned.info()}
+
+
+def copy_probe(mib):
+    from cfft_amd import check, lib
+    from cfft_amd.device import DeviceBuffer, Stream
+    nbytes = mib << 20
+    src = DeviceBuffer((nbytes // 8,), np.uint64)
+    dst = DeviceBuffer((nbytes // 8,), np.uint64)
+    s = Stream()
+    ms = time_runs(lambda: check(lib.cfft_memcpy_dtod(dst.dptr, src.dptr, nbytes, s.handle)), s,
+                   1, 1)
+    return {"mode": "copy", "bytes": nbytes, "ms": round(ms[0], 4)}
 
 
 def main():
@@ -79,13 +172,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain-polys", type=int, default=512,
                     help="products timed in the one-call-per-polynomial form")
-    ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="HBM bytes per launch measured with rocprofv3 --pmc, if known")
+    ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
+                    help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
+    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "copy"], default="all")
+    ap.add_argument("--uneven-steps", type=int, default=20)
+    ap.add_argument("--copy-mib", type=int, default=128)
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build targets one GPU; --gpus %d is not supported (the batch "
-              "shards independently across GPUs, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported (products are "
+              "independent, so a batch would shard across GPUs with no exchange)" % args.gpus,
+              file=sys.stderr)
         return 2
 
     import cfft_amd
@@ -96,6 +193,12 @@ def main():
     if cfft_amd.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+    if args.mode == "copy":
+        print(json.dumps(copy_probe(args.copy_mib)))
+        return 0
+    if args.mode == "uneven":
+        print(json.dumps({"mode": "uneven", "uneven": bench_uneven(args.uneven_steps, 2)}))
+        return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
     s = Stream()
@@ -123,16 +226,21 @@ def main():
 
     # plain form: one call per polynomial, same kernel, same inputs
     npl = min(args.plain_polys, BATCH)
-    s.synchronize()
-    t1 = time.perf_counter()
-    for i in range(npl):
-        off = i * N * 8
-        cfft_amd.check(cfft_amd.lib.cfft_negacyclic_polymul_u64(
-            plan.handle, da.dptr + off, db.dptr + off, dc.dptr + off, 1, s.handle))
-    s.synchronize()
-    plain_rate = npl / (time.perf_counter() - t1)
-    c_plain = dc.download()
-    plain_equal = bool(np.array_equal(c_plain[:npl], c_gpu[:npl]))
+    plain = None
+    if npl > 0:
+        s.synchronize()
+        t1 = time.perf_counter()
+        for i in range(npl):
+            off = i * N * 8
+            cfft_amd.check(cfft_amd.lib.cfft_negacyclic_polymul_u64(
+                plan.handle, da.dptr + off, db.dptr + off, dc.dptr + off, 1, s.handle))
+        s.synchronize()
+        plain_rate = npl / (time.perf_counter() - t1)
+        c_plain = dc.download()
+        plain = {"value": round(plain_rate, 1), "unit": "poly-mults/s",
+                 "form": "one cfft_negacyclic_polymul_u64 call per polynomial (%d polys)" % npl,
+                 "batched_over_plain": round(value / plain_rate, 2),
+                 "identical_results": bool(np.array_equal(c_plain[:npl], c_gpu[:npl]))}
 
     achieved = BYTES_PER_PRODUCT * BATCH / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
@@ -150,12 +258,12 @@ def main():
                                   "digits in [-128,127], unordered plan %s base_n=%d"
                                   % (FftAlgo(args.algo).name, args.base_n),
                       "N": N, "batch": BATCH},
-           "plain": {"value": round(plain_rate, 1), "unit": "poly-mults/s",
-                     "form": "one cfft_negacyclic_polymul_u64 call per polynomial (%d polys)" % npl,
-                     "batched_over_plain": round(value / plain_rate, 2),
-                     "identical_results": plain_equal}}
+           "plain": plain}
 
-    if not args.no_cpu:
+    if args.mode == "all":
+        out["uneven"] = bench_uneven(args.uneven_steps, 2)
+
+    if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
         out["cpu_baseline"] = cpu
         out["gpu_equals_cpu_bitwise"] = bool(np.array_equal(c_cpu, c_gpu[:ns]))

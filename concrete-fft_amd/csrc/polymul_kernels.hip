@@ -181,8 +181,9 @@ template <int LOG2N>
 hipError_t launch_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                           const dc64 *tw, const dc64 *twist, hipStream_t s) {
   using S = R16<LOG2N>;
-  if (LOG2N == 10 && getenv("CFFT_R16_GRID") == nullptr)
-    return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
+  // n = 1024: one product per wave, 8 waves per workgroup sharing the
+  // twiddles staged in LDS (measured faster than one workgroup per product)
+  if (LOG2N == 10) return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
   const uint32_t grid = (uint32_t)batch;
   hipLaunchKernelGGL(k_r16_polymul<LOG2N>, dim3(grid), dim3(S::T), 0, s, a, b, c, batch,
                      reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
