@@ -32,6 +32,22 @@ class CfftPanic(RuntimeError):
         self.status = status
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("pinned_bytes", ctypes.c_size_t),
+                ("max_batch", ctypes.c_uint32), ("streams", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("key_id", ctypes.c_int), ("N", ctypes.c_size_t),
+                ("in0", ctypes.c_void_p), ("in1", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
+class ServerStatistics(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("admission_waits", ctypes.c_uint64), ("max_inflight", ctypes.c_uint64),
+                ("hbm_budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_uint32)]
+
+
 # (name, restype, argtypes)
 _SIGS = [
     ("cfft_status_string", ctypes.c_char_p, [c_int]),
@@ -94,6 +110,15 @@ _SIGS = [
     ("cfft_uneven_run_plain", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("cfft_uneven_class_algo", c_int, [c_vp, ctypes.c_uint32, ctypes.POINTER(c_int),
                                        ctypes.POINTER(c_size)]),
+    ("cfft_server_create", c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_vp)]),
+    ("cfft_server_destroy", c_int, [c_vp]),
+    ("cfft_server_add_key", c_int, [c_vp, c_size, c_int, c_int, c_int, c_vp,
+                                    ctypes.POINTER(c_int)]),
+    ("cfft_server_submit", c_int, [c_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]),
+    ("cfft_server_poll", c_int, [c_vp, c_u64, ctypes.POINTER(c_int)]),
+    ("cfft_server_wait", c_int, [c_vp, c_u64]),
+    ("cfft_server_replay", c_int, [c_vp, ctypes.POINTER(Request), c_size, c_vp, c_vp]),
+    ("cfft_server_stats", c_int, [c_vp, ctypes.POINTER(ServerStatistics)]),
     ("cfft_uneven_info", c_int, [c_vp, ctypes.POINTER(c_size), ctypes.POINTER(c_size),
                                  ctypes.POINTER(c_size), ctypes.POINTER(ctypes.c_uint32)]),
 ]

@@ -88,8 +88,6 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
   if (base_log < 1 || base_log > 32 || levels * base_log > 64)
     return set_error(CFFT_ERROR_INVALID_ARGUMENT,
                      "gadget base_log must be in [1, 32] with levels * base_log <= 64");
-  if (batch == 0) return CFFT_SUCCESS;
-  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   int dev = 0, maxb = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
@@ -98,6 +96,8 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
   if (need > (size_t)maxb)
     return set_error(CFFT_ERROR_SIZE, "external product: k1 Fourier accumulators of this size do "
                                       "not fit in one workgroup's LDS");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   return hip_error(cfft_launch::external_product(
                        plan->fwd_full, plan->inv_full,
                        reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,

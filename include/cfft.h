@@ -268,6 +268,63 @@ int cfft_uneven_class_algo(cfft_uneven u, uint32_t log2_N, int *algo, size_t *ba
 int cfft_uneven_info(cfft_uneven u, size_t *njobs, size_t *ntiles, size_t *nclasses,
                      uint32_t *grid);
 
+/* ---- request server (no counterpart in the reference): continuous batching
+ *      of negacyclic products and external products on one device ---- */
+/* A dispatcher thread groups waiting requests by (kind, N, key) in arrival
+ * order and launches a batch whenever one of `streams` launch slots is free
+ * and the device pool (hbm_budget_bytes, carved once from free device
+ * memory) and the pinned staging pool have room for it; requests that do not
+ * fit wait for running batches to release memory.  Batches on different
+ * streams overlap on the device.  Each result is bit-identical to the same
+ * request computed alone (the batched kernels treat products independently). */
+typedef struct {
+  size_t hbm_budget_bytes; /* 0: min(1 GiB, free device memory / 4) */
+  size_t pinned_bytes;     /* 0: same as hbm_budget_bytes */
+  uint32_t max_batch;      /* requests per launch, 0: 1024 */
+  uint32_t streams;        /* launch slots, 0: 4 (at most 16) */
+} cfft_server_config;
+typedef enum { CFFT_REQUEST_POLYMUL = 0, CFFT_REQUEST_EXTERNAL_PRODUCT = 1 } cfft_request_kind;
+/* Host buffers, owned by the caller and valid until the request completes.
+ * POLYMUL: in0 = A (N u64 torus), in1 = B (N i64), out = N u64 (as
+ * cfft_negacyclic_polymul_u64 with a Dif16 plan, base_n = min(N/2, 1024)).
+ * EXTERNAL_PRODUCT: key_id from cfft_server_add_key, in0 = GLWE (k1 x N
+ * u64), out = k1 x N u64 (as cfft_external_product). */
+typedef struct {
+  int kind;
+  int key_id;
+  size_t N;
+  const void *in0;
+  const void *in1;
+  void *out;
+} cfft_request;
+typedef struct {
+  uint64_t batches;
+  uint64_t completed;
+  uint64_t admission_waits; /* batch formations deferred for lack of free memory */
+  uint64_t max_inflight;
+  size_t hbm_budget_bytes;
+  uint32_t streams;
+} cfft_server_statistics;
+typedef struct cfft_server_s *cfft_server;
+int cfft_server_create(const cfft_server_config *config, cfft_server *out);
+/* Drains submitted requests, then stops the dispatcher and frees everything. */
+int cfft_server_destroy(cfft_server s);
+/* Registers a GGSW key (host, [row][col][N] u64 torus) for external products
+ * of size N; it is transformed once and stays resident in the Fourier domain. */
+int cfft_server_add_key(cfft_server s, size_t N, int k1, int levels, int base_log,
+                        const uint64_t *key, int *key_id);
+/* Non-blocking; CFFT_ERROR_BUSY if the request alone exceeds the budget. */
+int cfft_server_submit(cfft_server s, const cfft_request *request, uint64_t *ticket);
+int cfft_server_poll(cfft_server s, uint64_t ticket, int *done);
+/* Blocks until the request is done and its output written; forgets the ticket. */
+int cfft_server_wait(cfft_server s, uint64_t ticket);
+/* Open-loop replay: submits request i at arrival_s[i] seconds after the start
+ * (from this thread), waits for all, and stores each latency from its
+ * scheduled arrival to its completion in latency_s[i]. */
+int cfft_server_replay(cfft_server s, const cfft_request *requests, size_t n,
+                       const double *arrival_s, double *latency_s);
+int cfft_server_stats(cfft_server s, cfft_server_statistics *stats);
+
 #ifdef __cplusplus
 }
 #endif

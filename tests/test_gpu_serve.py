@@ -1,0 +1,130 @@
+"""Request server on the GPU: every result is bit-identical alone and in a
+mixed burst, equals the direct batched entry points, survives admission
+pressure (a budget of a few requests), and an open-loop Poisson replay
+completes with correct results."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+M = 2**64
+K1, LEVELS, BASE_LOG = 2, 3, 8
+
+
+def _setup(server, rng, sizes=(1024, 2048, 4096)):
+    keys = {}
+    for N in sizes:
+        key = rng.integers(0, M, (K1 * LEVELS, K1, N), dtype=np.uint64)
+        keys[N] = (server.add_key(key, K1, LEVELS, BASE_LOG), key)
+    return keys
+
+
+def _requests(server, keys, rng, count):
+    reqs = []
+    for i in range(count):
+        if i % 4 == 3:
+            N = int(rng.choice([1024, 2048]))
+            a = rng.integers(0, M, N, dtype=np.uint64)
+            b = rng.integers(-128, 128, N, dtype=np.int64)
+            reqs.append(("polymul", N, (a, b)))
+        else:
+            N = int(rng.choice(sorted(keys)))
+            reqs.append(("extprod", N, rng.integers(0, M, (K1, N), dtype=np.uint64)))
+    return reqs
+
+
+def _make(server, keys, r):
+    kind, N, data = r
+    if kind == "polymul":
+        return server.polymul_request(*data)
+    return server.extprod_request(keys[N][0], data)
+
+
+def test_alone_equals_mix_equals_direct(cfft):
+    from cfft_amd import products
+    from cfft_amd.serve import Server
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(5)
+    srv = Server()
+    keys = _setup(srv, rng)
+    reqs = _requests(srv, keys, rng, 48)
+    alone = []
+    for r in reqs:
+        q, keep = _make(srv, keys, r)
+        alone.append(srv.wait(srv.submit(q, keep)).copy())
+    tickets = []
+    for r in reqs:
+        q, keep = _make(srv, keys, r)
+        tickets.append(srv.submit(q, keep))
+    mix = [srv.wait(t) for t in tickets]
+    for i, (x, y) in enumerate(zip(alone, mix)):
+        assert np.array_equal(x, y), i
+    st = srv.stats()
+    assert st["completed"] == 96 and st["batches"] < 96
+    # direct entry points, same plans (Dif16, base_n = min(N/2, 1024))
+    for i, (kind, N, data) in enumerate(reqs):
+        plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, min(N // 2, 1024)))
+        if kind == "polymul":
+            want = products.negacyclic_polymul_u64_host(plan, data[0][None], data[1][None])[0]
+        else:
+            from cfft_amd.device import DeviceBuffer
+            key = keys[N][1]
+            dk = DeviceBuffer.from_host(key)
+            kf = DeviceBuffer((K1 * LEVELS, K1, N // 2), np.complex128)
+            products.ggsw_to_fourier(plan, dk, K1, LEVELS, kf)
+            dg = DeviceBuffer.from_host(data[None])
+            do = DeviceBuffer((1, K1, N), np.uint64)
+            products.external_product(plan, kf, K1, LEVELS, BASE_LOG, dg, do, 1)
+            want = do.download()[0]
+        assert np.array_equal(alone[i], want), (i, kind, N)
+    srv.close()
+
+
+def test_admission_under_small_budget(cfft):
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(6)
+    # room for two N = 4096 external products (64 KiB in + 64 KiB out each)
+    budget = 2 * 2 * 65536 + 4096
+    srv = Server(hbm_budget_bytes=budget, pinned_bytes=budget)
+    keys = _setup(srv, rng, sizes=(4096,))
+    reqs = [("extprod", 4096, rng.integers(0, M, (K1, 4096), dtype=np.uint64)) for _ in range(40)]
+    alone = []
+    for r in reqs[:5]:
+        q, keep = _make(srv, keys, r)
+        alone.append(srv.wait(srv.submit(q, keep)).copy())
+    tickets = []
+    for r in reqs:
+        q, keep = _make(srv, keys, r)
+        tickets.append(srv.submit(q, keep))
+    outs = [srv.wait(t) for t in tickets]
+    for i in range(5):
+        assert np.array_equal(outs[i], alone[i])
+    st = srv.stats()
+    assert st["completed"] == 45
+    # a request larger than the whole budget (N = 16384 product: 384 KiB) is
+    # refused at submission
+    from cfft_amd import CfftPanic
+    big = srv.polymul_request(np.zeros(16384, np.uint64), np.zeros(16384, np.int64))
+    with pytest.raises(CfftPanic) as ei:
+        srv.submit(*big)
+    assert ei.value.status == 14
+    srv.close()
+
+
+def test_poisson_replay(cfft):
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(7)
+    srv = Server()
+    keys = _setup(srv, rng)
+    reqs = _requests(srv, keys, rng, 2000)
+    made = [_make(srv, keys, r) for r in reqs]
+    arrivals = np.cumsum(rng.exponential(1.0 / 20000.0, len(reqs)))
+    lat = srv.replay([q for q, _ in made], arrivals)
+    assert np.all(lat > 0) and np.all(np.isfinite(lat))
+    # spot-check results against the same requests served alone
+    for i in rng.choice(len(reqs), 20, replace=False):
+        q, keep = _make(srv, keys, reqs[i])
+        alone = srv.wait(srv.submit(q, keep))
+        assert np.array_equal(made[i][1][-1], alone), i
+    print("p50 %.1f us p99 %.1f us" % (np.percentile(lat, 50) * 1e6, np.percentile(lat, 99) * 1e6))
+    srv.close()
