@@ -20,8 +20,13 @@ negacyclic round-trip jobs (fold, twist, forward, inverse, untwist) with N
 drawn from {512, ..., 16384} with weights {.10, .20, .35, .20, .10, .05},
 values uniform in [-1, 1), as one persistent launch replayed from a HIP graph
 with autotuned per-size plans, against the fixed-schedule plain form (Dif4,
-one launch per size class).  `--mode copy` runs one device-to-device copy of
---copy-mib MiB (the reference kernel for scaling HBM byte counters).
+one launch per size class).  `serve` reports the request server under
+open-loop Poisson arrivals of external products (k+1 = 2 GLWE polynomials of
+N in {1024, 2048, 4096}, l = 3 levels of base 2^8, resident Fourier-domain
+keys) at 5k, 20k and 50k requests/s: p50 / p99 latency from scheduled
+arrival to result, achieved requests/s, batching.  `--mode copy` runs one
+device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
+byte counters).
 """
 import argparse
 import json
@@ -150,6 +155,66 @@ def bench_uneven(steps, warmup):
         "roundtrip_error_max": err, "schedule": tuned.info()}
 
 
+SERVE_RATES = [5000, 20000, 50000]
+SERVE_SIZES = [1024, 2048, 4096]
+
+
+def bench_serve(duration_s, rates=SERVE_RATES):
+    """Open-loop Poisson replay of external-product requests (k+1 = 2, l = 3,
+    base 2^8, N uniform over SERVE_SIZES) against resident Fourier keys."""
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(5)
+    srv = Server()
+    keys = {}
+    for N in SERVE_SIZES:
+        key = rng.integers(0, 2**64, (6, 2, N), dtype=np.uint64)
+        keys[N] = srv.add_key(key, 2, 3, 8)
+    glwes = {N: rng.integers(0, 2**64, (64, 2, N), dtype=np.uint64) for N in SERVE_SIZES}
+
+    def trace(rate, count):
+        Ns = rng.choice(SERVE_SIZES, count)
+        made = [srv.extprod_request(keys[int(N)], glwes[int(N)][i % 64])
+                for i, N in enumerate(Ns)]
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+        return made, arrivals
+
+    # warm-up at every rate: the first passes of a process show one-time
+    # stalls of 10-20 ms (runtime resources growing with the number of
+    # batches in flight); steady-state serving is what is measured
+    for rate in rates:
+        made, arr = trace(rate, max(200, int(rate * 0.2)))
+        srv.replay([q for q, _ in made], arr)
+    out = []
+    for rate in rates:
+        count = max(200, int(rate * duration_s))
+        made, arr = trace(rate, count)
+        before = srv.stats()
+        lat = srv.replay([q for q, _ in made], arr)
+        after = srv.stats()
+        span = float(arr[-1] + lat[-1])
+        # results of the mix against the same request alone, on a sample
+        same = True
+        for i in rng.choice(count, 8, replace=False):
+            q, keep = srv.extprod_request(made[i][0].key_id,
+                                          glwes[int(made[i][0].N)][i % 64])
+            alone = srv.wait(srv.submit(q, keep))
+            same &= bool(np.array_equal(alone, made[i][1][-1]))
+        nb = after["batches"] - before["batches"]
+        out.append({"offered_req_per_s": rate, "requests": count,
+                    "achieved_req_per_s": round(count / span, 1),
+                    "p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
+                    "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
+                    "max_us": round(float(lat.max()) * 1e6, 1),
+                    "batches": nb, "mean_batch": round(count / max(nb, 1), 2),
+                    "max_inflight": after["max_inflight"],
+                    "alone_equals_mix": same})
+    st = srv.stats()
+    srv.close()
+    return {"workload": "external products, GLWE k+1=2 of N in %s, l=3, base 2^8, resident "
+                        "Fourier keys, Poisson arrivals, %.2f s per rate" % (SERVE_SIZES, duration_s),
+            "rates": out, "hbm_budget_bytes": st["hbm_budget_bytes"], "streams": st["streams"]}
+
+
 def copy_probe(mib):
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -174,7 +239,8 @@ def main():
                     help="products timed in the one-call-per-polynomial form")
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
-    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "copy"], default="all")
+    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "copy"], default="all")
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=128)
     args = ap.parse_args()
@@ -198,6 +264,9 @@ def main():
         return 0
     if args.mode == "uneven":
         print(json.dumps({"mode": "uneven", "uneven": bench_uneven(args.uneven_steps, 2)}))
+        return 0
+    if args.mode == "serve":
+        print(json.dumps({"mode": "serve", "serve": bench_serve(args.serve_seconds)}))
         return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
@@ -262,6 +331,7 @@ def main():
 
     if args.mode == "all":
         out["uneven"] = bench_uneven(args.uneven_steps, 2)
+        out["serve"] = bench_serve(args.serve_seconds)
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
