@@ -24,7 +24,11 @@ one launch per size class).  `serve` reports the request server under
 open-loop Poisson arrivals of external products (k+1 = 2 GLWE polynomials of
 N in {1024, 2048, 4096}, l = 3 levels of base 2^8, resident Fourier-domain
 keys) at 5k, 20k and 50k requests/s: p50 / p99 latency from scheduled
-arrival to result, achieved requests/s, batching.  `--mode copy` runs one
+arrival to result, achieved requests/s, batching.  `exact` times the exact
+u64 product (both operands uniform over [0, 2^64), N = 2048 and 4096, batch
+2048) through f64 transforms of 16-bit limbs against the double-double
+transforms of 32-bit limbs, with both results checked equal and a sample
+checked against the u64 schoolbook.  `--mode copy` runs one
 device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
 byte counters).
 """
@@ -215,6 +219,52 @@ def bench_serve(duration_s, rates=SERVE_RATES):
             "rates": out, "hbm_budget_bytes": st["hbm_budget_bytes"], "streams": st["streams"]}
 
 
+EXACT_SIZES = [2048, 4096]
+EXACT_BATCH = 2048
+
+
+def bench_exact(steps, warmup):
+    """Exact u64 products, both operands uniform over [0, 2^64): the f64
+    16-bit-limb path against the double-double 32-bit-limb path, same inputs."""
+    from oracle_lib import load_oracle
+    from cfft_amd import fft128, products
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(4)
+    s = Stream()
+    out = []
+    for N in EXACT_SIZES:
+        a = rng.integers(0, 2**64, (EXACT_BATCH, N), dtype=np.uint64)
+        b = rng.integers(0, 2**64, (EXACT_BATCH, N), dtype=np.uint64)
+        da, db = DeviceBuffer.from_host(a, s), DeviceBuffer.from_host(b, s)
+        c1, c2 = DeviceBuffer((EXACT_BATCH, N), np.uint64), DeviceBuffer((EXACT_BATCH, N), np.uint64)
+        m1, m2 = DeviceBuffer((EXACT_BATCH,), np.float64), DeviceBuffer((EXACT_BATCH,), np.float64)
+        p64 = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, min(N // 2, 1024)))
+        p128 = fft128.Plan(N // 2)
+        t64 = time_runs(lambda: products.exact_polymul_u64(p64, da, db, c1, EXACT_BATCH, m1, s),
+                        s, steps, warmup)
+        t128 = time_runs(lambda: products.exact128_polymul_u64(p128, da, db, c2, EXACT_BATCH, m2,
+                                                                s), s, steps, warmup)
+        r1, r2 = c1.download(), c2.download()
+        k = 4
+        exact = load_oracle().schoolbook_u64(a[:k], b[:k], threads=host_cores())
+        ms64, ms128 = float(np.median(t64)), float(np.median(t128))
+        bytes_algo = 24 * N * EXACT_BATCH
+        out.append({"N": N, "batch": EXACT_BATCH,
+                    "f64_limb16": {"products_per_s": round(EXACT_BATCH / (ms64 * 1e-3), 1),
+                                   "ms": round(ms64, 4),
+                                   "hbm_frac": round(bytes_algo / (ms64 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                   "max_margin": float(m1.download().max())},
+                    "fft128_limb32": {"products_per_s": round(EXACT_BATCH / (ms128 * 1e-3), 1),
+                                      "ms": round(ms128, 4),
+                                      "max_margin": float(m2.download().max())},
+                    "f64_over_fft128": round(ms128 / ms64, 3),
+                    "paths_identical": bool(np.array_equal(r1, r2)),
+                    "bit_exact_vs_schoolbook": bool(np.array_equal(r1[:k], exact)),
+                    "schoolbook_sample": k})
+    return out
+
+
 def copy_probe(mib):
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -239,7 +289,8 @@ def main():
                     help="products timed in the one-call-per-polynomial form")
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
-    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "copy"], default="all")
+    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "copy"],
+                    default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=128)
@@ -267,6 +318,9 @@ def main():
         return 0
     if args.mode == "serve":
         print(json.dumps({"mode": "serve", "serve": bench_serve(args.serve_seconds)}))
+        return 0
+    if args.mode == "exact":
+        print(json.dumps({"mode": "exact", "exact": bench_exact(10, 2)}))
         return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
@@ -332,6 +386,7 @@ def main():
     if args.mode == "all":
         out["uneven"] = bench_uneven(args.uneven_steps, 2)
         out["serve"] = bench_serve(args.serve_seconds)
+        out["exact"] = bench_exact(10, 2)
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)

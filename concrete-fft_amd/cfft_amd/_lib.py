@@ -101,6 +101,7 @@ _SIGS = [
     ("cfft128_init_negacyclic_twiddles", c_int, [c_size, c_vp, c_vp, c_vp, c_vp]),
     ("cfft_negacyclic_polymul_u64", c_int, [c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     ("cfft_exact_polymul_u64", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
+    ("cfft128_exact_polymul_u64", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     ("cfft_ggsw_to_fourier", c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     ("cfft_external_product", c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_size, c_vp]),
     ("cfft_uneven_create", c_int, [c_vp, c_size, c_size, c_size, c_size, c_int, c_int, c_dbl,

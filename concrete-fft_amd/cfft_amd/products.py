@@ -49,6 +49,28 @@ def exact_polymul_u64_host(plan, a, b, with_margin=False):
     return out
 
 
+def exact128_polymul_u64(plan128, a, b, c, batch, margin=None, stream=None):
+    """Exact C = A * B mod (X^N + 1, 2^64) over the double-double transform of
+    an fft128 plan of size N/2 (32-bit limbs)."""
+    check(lib.cfft128_exact_polymul_u64(plan128.handle, a.dptr, b.dptr, c.dptr,
+                                        margin.dptr if margin is not None else None, batch,
+                                        _h(stream)))
+
+
+def exact128_polymul_u64_host(plan128, a, b, with_margin=False):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    batch = a.shape[0] if a.ndim == 2 else 1
+    da, db = DeviceBuffer.from_host(a), DeviceBuffer.from_host(b)
+    dc = DeviceBuffer(a.shape, np.uint64)
+    dm = DeviceBuffer((batch,), np.float64) if with_margin else None
+    exact128_polymul_u64(plan128, da, db, dc, batch, dm)
+    out = dc.download()
+    if with_margin:
+        return out, dm.download()
+    return out
+
+
 def ggsw_to_fourier(plan, key, k1, levels, key_fourier, stream=None):
     check(lib.cfft_ggsw_to_fourier(plan.handle, key.dptr, k1, levels, key_fourier.dptr,
                                    _h(stream)))

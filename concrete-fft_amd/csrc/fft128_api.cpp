@@ -16,6 +16,9 @@ namespace cfft_launch {
 hipError_t fft128_run(bool fwd, int mulv, uint32_t L, double *re0, double *re1, double *im0,
                       double *im1, const double *t_re0, const double *t_re1,
                       const double *t_im0, const double *t_im1, uint64_t batch, hipStream_t s);
+hipError_t exact128_polymul(uint32_t L, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                            double *margin, uint64_t batch, const double *t, double *S,
+                            hipStream_t s);
 }
 
 namespace {
@@ -143,6 +146,24 @@ int cfft128_init_negacyclic_twiddles(size_t n, double *re0, double *re1, double 
   if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
   cfft_host::negacyclic_twiddles128(n, re0, re1, im0, im1);
   return CFFT_SUCCESS;
+}
+
+int cfft128_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                              double *margin, size_t batch, cfft_stream_t stream) {
+  if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
+  if (plan->kind != cfft_plan_s::FFT128)
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "not an fft128 plan");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!a || !b || !c) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = plan->n;
+  double *S = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&S), 16 * batch * n * sizeof(double), s);
+  if (e != hipSuccess) return hip_error(e, "hipMallocAsync (fft128 product scratch)");
+  e = cfft_launch::exact128_polymul(ilog2(n), a, b, c, margin, batch, plan->d_tw128, S, s);
+  hipError_t e2 = hipFreeAsync(S, s);
+  if (e != hipSuccess) return hip_error(e, "fft128 exact product");
+  return hip_error(e2, "hipFreeAsync");
 }
 
 }  // extern "C"

@@ -277,6 +277,140 @@ __global__ void __launch_bounds__(512) k_fft128(double *re0, double *re1, double
   }
 }
 
+// ---- exact u64 product over the double-double transform ----
+//
+// C = A * B mod (X^N + 1, 2^64) with A = a0 + 2^32 a1 (balanced 32-bit limbs,
+// likewise B): C = a0 (x) b0 + 2^32 (a0 (x) b1 + a1 (x) b0) mod 2^64.  Each
+// limb product has |coefficients| <= N 2^62 <= 2^76 for N <= 8192, inside the
+// 106-bit double-double mantissa, so rounding the transform result to the
+// nearest integer recovers it exactly while the transform error stays below
+// 1/2 (the largest distance seen is reported as the margin).  The fold, the
+// (2/N) factor and the layout follow the reference's fft128 product test
+// (src/fft128/mod.rs:1996-2058).
+//
+// Scratch S holds 4 SoA arrays of 4*batch transforms of n points:
+// S[arr][(t * batch + p) * n + j], t = a0, a1, b0, b1; after the pointwise
+// step t = 0 holds a0 b0 and t = 1 holds a0 b1 + a1 b0.
+
+__device__ __forceinline__ void limbs32(uint64_t v, double &lo, double &hi) {
+  int64_t d0 = (int64_t)(v & 0xffffffffull);
+  const int64_t carry = d0 >= ((int64_t)1 << 31) ? 1 : 0;
+  d0 -= carry << 32;
+  // the top limb only matters mod 2^32: balance it too
+  int64_t d1 = (int64_t)(v >> 32) + carry;
+  d1 = ((d1 + ((int64_t)1 << 31)) & 0xffffffffll) - ((int64_t)1 << 31);
+  lo = (double)d0;
+  hi = (double)d1;
+}
+
+__global__ void __launch_bounds__(256) k_exact128_split(const uint64_t *__restrict__ A,
+                                                        const uint64_t *__restrict__ B,
+                                                        double *__restrict__ S, double *margin,
+                                                        uint64_t batch, uint32_t L) {
+  const uint32_t n = 1u << L;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= batch * n) return;
+  const uint64_t p = gid >> L;
+  const uint32_t j = (uint32_t)(gid & (n - 1));
+  if (margin && j == 0) margin[p] = 0.0;
+  const uint64_t plane = 4 * batch * (uint64_t)n;  // one SoA array
+  double *re0 = S, *re1 = S + plane, *im0 = S + 2 * plane, *im1 = S + 3 * plane;
+  const uint64_t *ab[2] = {A + p * 2 * n, B + p * 2 * n};
+#pragma unroll
+  for (int o = 0; o < 2; ++o) {
+    double r_lo, r_hi, i_lo, i_hi;
+    limbs32(ab[o][j], r_lo, r_hi);
+    limbs32(ab[o][j + n], i_lo, i_hi);
+    const uint64_t t0 = ((uint64_t)(2 * o) * batch + p) * n + j;
+    const uint64_t t1 = ((uint64_t)(2 * o + 1) * batch + p) * n + j;
+    re0[t0] = r_lo;
+    re1[t0] = 0.0;
+    im0[t0] = i_lo;
+    im1[t0] = 0.0;
+    re0[t1] = r_hi;
+    re1[t1] = 0.0;
+    im0[t1] = i_hi;
+    im1[t1] = 0.0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_exact128_pointwise(double *__restrict__ S,
+                                                            uint64_t batch, uint32_t L) {
+  const uint32_t n = 1u << L;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= batch * n) return;
+  const uint64_t plane = 4 * batch * (uint64_t)n;
+  double *re0 = S, *re1 = S + plane, *im0 = S + 2 * plane, *im1 = S + 3 * plane;
+  cdd z[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const uint64_t i = (uint64_t)t * batch * n + gid;
+    z[t].re = dd{re0[i], re1[i]};
+    z[t].im = dd{im0[i], im1[i]};
+  }
+  const double f = 2.0 / (double)(2u * n);  // 1/n, exact
+  const cdd p0 = cmul<1>(z[0], z[2]);
+  const cdd q0 = cmul<1>(z[0], z[3]), q1 = cmul<1>(z[1], z[2]);
+  cdd p1;
+  p1.re = add_est(q0.re, q1.re);
+  p1.im = add_est(q0.im, q1.im);
+  re0[gid] = p0.re.hi * f;
+  re1[gid] = p0.re.lo * f;
+  im0[gid] = p0.im.hi * f;
+  im1[gid] = p0.im.lo * f;
+  const uint64_t i1 = batch * n + gid;
+  re0[i1] = p1.re.hi * f;
+  re1[i1] = p1.re.lo * f;
+  im0[i1] = p1.im.hi * f;
+  im1[i1] = p1.im.lo * f;
+}
+
+// integer-valued double -> its value mod 2^64 (any magnitude)
+__device__ __forceinline__ uint64_t int_double_mod64(double x) {
+  if (fabs(x) < 9223372036854775808.0) return (uint64_t)(int64_t)x;
+  const uint64_t bits = (uint64_t)__double_as_longlong(x);
+  const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;  // >= 1086 here
+  const uint64_t m = (bits & 0x000fffffffffffffull) | 0x0010000000000000ull;
+  const uint32_t sh = ex - 1075u;
+  const uint64_t r = sh >= 64u ? 0ull : (m << sh);
+  return (bits >> 63) ? (0ull - r) : r;
+}
+
+// nearest integer of hi + lo, mod 2^64, and its distance
+__device__ __forceinline__ uint64_t dd_round_mod64(double hi, double lo, double &dist) {
+  const double rh = round(hi);
+  const double f = (hi - rh) + lo;  // hi - rh is exact
+  const double rl = round(f);
+  dist = fabs(f - rl);
+  return int_double_mod64(rh) + (uint64_t)(int64_t)rl;
+}
+
+__global__ void __launch_bounds__(256) k_exact128_combine(const double *__restrict__ S,
+                                                          uint64_t *__restrict__ C,
+                                                          double *margin, uint64_t batch,
+                                                          uint32_t L) {
+  const uint32_t n = 1u << L;
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= batch * n) return;
+  const uint64_t p = gid >> L;
+  const uint32_t j = (uint32_t)(gid & (n - 1));
+  const uint64_t plane = 4 * batch * (uint64_t)n;
+  const double *re0 = S, *re1 = S + plane, *im0 = S + 2 * plane, *im1 = S + 3 * plane;
+  const uint64_t i1 = batch * n + gid;
+  double d0, d1, d2, d3;
+  const uint64_t c_re = dd_round_mod64(re0[gid], re1[gid], d0) +
+                        (dd_round_mod64(re0[i1], re1[i1], d1) << 32);
+  const uint64_t c_im = dd_round_mod64(im0[gid], im1[gid], d2) +
+                        (dd_round_mod64(im0[i1], im1[i1], d3) << 32);
+  C[p * 2 * n + j] = c_re;
+  C[p * 2 * n + j + n] = c_im;
+  if (margin) {
+    const double w = fmax(fmax(d0, d1), fmax(d2, d3));
+    atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
+              (unsigned long long)__double_as_longlong(w));
+  }
+}
+
 hipError_t set_lds(const void *kernel, size_t bytes) {
   static std::mutex mu;
   static std::set<const void *> done;
@@ -323,6 +457,32 @@ hipError_t fft128_run(bool fwd, int mulv, uint32_t L, double *re0, double *re1, 
     if (mulv) { CFFT128_LAUNCH(false, 1) } else { CFFT128_LAUNCH(false, 0) }
   }
 #undef CFFT128_LAUNCH
+  return hipGetLastError();
+}
+
+// S: 16 * batch * n doubles; margin (batch doubles) may be null.  twiddles
+// t: 4 arrays of n.
+hipError_t exact128_polymul(uint32_t L, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                            double *margin, uint64_t batch, const double *t, double *S,
+                            hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const uint32_t n = 1u << L;
+  const uint64_t total = batch * n;
+  const dim3 grid((uint32_t)((total + 255) / 256)), block(256);
+  const uint64_t plane = 4 * batch * (uint64_t)n;
+  hipLaunchKernelGGL(k_exact128_split, grid, block, 0, s, a, b, S, margin, batch, L);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = fft128_run(true, 1, L, S, S + plane, S + 2 * plane, S + 3 * plane, t, t + n, t + 2 * n,
+                 t + 3 * n, 4 * batch, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_exact128_pointwise, grid, block, 0, s, S, batch, L);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  e = fft128_run(false, 1, L, S, S + plane, S + 2 * plane, S + 3 * plane, t, t + n, t + 2 * n,
+                 t + 3 * n, 2 * batch, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_exact128_combine, grid, block, 0, s, S, c, margin, batch, L);
   return hipGetLastError();
 }
 

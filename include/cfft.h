@@ -209,6 +209,13 @@ int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t
  * the result is exact while it stays < 0.5. */
 int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
                            double *margin, size_t batch, cfft_stream_t stream);
+/* The same exact product over the double-double transform of an fft128 plan
+ * of fft_size N/2 (the product the reference's fft128 test computes,
+ * src/fft128/mod.rs:1996-2058): balanced 32-bit limbs, 4 forward transforms,
+ * a0 b0 and a0 b1 + a1 b0 in the Fourier domain, 2 inverse transforms,
+ * rounding and carries mod 2^64.  Exact for N <= 8192; `margin` as above. */
+int cfft128_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
+                              double *margin, size_t batch, cfft_stream_t stream);
 
 /* ---- external product (TFHE GGSW x GLWE; caller-side in the reference) ---- */
 /* Transform a GGSW key of k1*levels rows x k1 columns of u64 torus

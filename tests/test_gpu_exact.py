@@ -58,3 +58,34 @@ def test_exact_independent_of_plan(cfft):
     outs = [products.exact_polymul_u64_host(Plan(N // 2, Method.UserProvided(FftAlgo(al), bn)), a, b)
             for al, bn in [(6, 512), (1, 32), (4, 128)]]
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+@pytest.mark.parametrize("N", [64, 256, 2048, 4096, 8192])
+def test_exact128_bit_exact(cfft, oracle, N):
+    # the double-double path: 32-bit limbs over an fft128 plan of size N/2
+    from cfft_amd import fft128, products
+    rng = np.random.default_rng(N + 128)
+    batch = 5
+    a = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    b = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    a[0, :3] = [M - 1, 1 << 63, (1 << 32) - 1]
+    b[0, :3] = [M - 1, (1 << 31), (1 << 63) + (1 << 31)]
+    plan = fft128.Plan(N // 2)
+    c, margin = products.exact128_polymul_u64_host(plan, a, b, with_margin=True)
+    assert np.array_equal(c, oracle.schoolbook_u64(a, b))
+    assert margin.max() < 0.5 and margin.min() >= 0.0
+    print("N=%d fft128 path max rounding margin %.3g" % (N, margin.max()))
+
+
+def test_exact_paths_agree_on_edge_values(cfft):
+    from cfft_amd import fft128, products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 4096
+    a = np.stack([np.full(N, M - 1, np.uint64), np.full(N, 1 << 63, np.uint64),
+                  np.full(N, 0x80000000, np.uint64)])
+    b = np.stack([np.full(N, M - 1, np.uint64), np.full(N, 1 << 63, np.uint64),
+                  np.full(N, 0x7fffffff80000000, np.uint64)])
+    c64 = products.exact_polymul_u64_host(Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024)),
+                                          a, b)
+    c128 = products.exact128_polymul_u64_host(fft128.Plan(N // 2), a, b)
+    assert np.array_equal(c64, c128)
