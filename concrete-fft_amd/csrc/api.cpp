@@ -57,6 +57,7 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_pos) (void)hipFree(d_pos);
   if (d_twist) (void)hipFree(d_twist);
   if (d_tw_r16) (void)hipFree(d_tw_r16);
+  if (d_tw_r8) (void)hipFree(d_tw_r8);
   if (d_untwist) (void)hipFree(d_untwist);
   if (d_tw128) (void)hipFree(d_tw128);
 }
@@ -202,6 +203,16 @@ int build_f64_plan(cfft_plan_s *p) {
     CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
     CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
   }
+  if (p->algo == CFFT_DIF8 && p->base_n == n && cfft_launch::r8_supported(log2n)) {
+    std::vector<cfft_host::C64> blk(8 * 128 + 8 * 16 + 8 * 2);
+    for (size_t k = 0; k < 8; ++k) {
+      for (size_t l = 0; l < 128; ++l) blk[k * 128 + l] = tw[(k * l) % n];
+      for (size_t q = 0; q < 16; ++q) blk[1024 + k * 16 + q] = tw[(8 * k * q) % n];
+      for (size_t q = 0; q < 2; ++q) blk[1024 + 128 + k * 2 + q] = tw[(64 * k * q) % n];
+    }
+    CFFT_HIP(hipMalloc(&p->d_tw_r8, blk.size() * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(p->d_tw_r8, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
+  }
   const unsigned nbits = log2n, bbits = ilog2(p->base_n);
   std::vector<uint32_t> freq(n);
   p->pos_of_freq.resize(n);
@@ -241,6 +252,9 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
     return hip_error(cfft_launch::r16_transform(log2n, fwd, d, batch, stride, p->d_tw_r16, s),
                      "r16_transform");
+  if (p->d_tw_r8)
+    return hip_error(cfft_launch::r8_transform(fwd, d, batch, stride, p->d_tw_r8, s),
+                     "r8_transform");
   if (fwd) {
     for (const Pass &P : p->fwd_global)
       CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
@@ -687,6 +701,9 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
     return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw_r16, p->d_twist, s),
                      "r16_polymul");
+  if (p->d_tw_r8)
+    return hip_error(cfft_launch::r8_polymul(a, b, c, batch, p->d_tw_r8, p->d_twist, s),
+                     "r8_polymul");
   return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
                                                 p->d_twist, p->d_untwist, s),
                    "polymul");

@@ -224,29 +224,40 @@ __device__ __forceinline__ constexpr int bitrev_digit(int k) {
   else return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1);
 }
 
-// ---- torus conversions (caller-side convention, see cfft.h) ----
-__device__ __forceinline__ double torus_to_f64(uint64_t a) {
-  return (double)(int64_t)a * 5.421010862427522e-20;  // 2^-64
+// ---- integer and torus conversions (caller-side convention) ----
+// (double)v for any int64 v, correctly rounded: hi 2^32 + lo is formed
+// exactly inside one fma and rounded once.
+__device__ __forceinline__ double i64_to_f64(int64_t v) {
+  const double hi = (double)(int32_t)(v >> 32);
+  const double lo = (double)(uint32_t)v;
+  return __fma_rn(hi, 4294967296.0, lo);
 }
-// Nearest multiple of 2^-64 of frac(x), as a u64 (fract in [-1/2, 1/2]; the
-// value 2^63 wraps to the same torus element as -2^63).  Reference formula:
-//   fract = x - round(x); v = round(fract * 2^64); (u64)(i64)v.
-// Fast path, same bits: when ulp(x) >= 2^-64 (biased exponent >= 1011),
-// x * 2^64 is an integer, fract * 2^64 = x * 2^64 - round(x) * 2^64 exactly,
-// and the result is x * 2^64 mod 2^64: the mantissa shifted left, negated
-// for negative x.  Otherwise |x| < 2^-11, round(x) = 0 and the formula is
-// applied directly.
+// (double)(int64)a * 2^-64: the same single rounding, scaled exactly
+// (the result is a normal number for a != 0).
+__device__ __forceinline__ double torus_to_f64(uint64_t a) {
+  const double hi = (double)(int32_t)(a >> 32);
+  const double lo = (double)(uint32_t)a * 5.421010862427522e-20;  // 2^-64, exact
+  return __fma_rn(hi, 2.3283064365386963e-10, lo);                // 2^-32
+}
+// Nearest multiple of 2^-64 of frac(x), as a u64.  Reference formula (half
+// away from zero in both roundings):
+//   fract = x - round(x); v = round(fract * 2^64); (u64)(i64)v, v = 2^63 -> 2^63.
+// Branch-free form with the same bits for every finite x:
+//   t = x - rint(x) is exact; it differs from fract only at half-integers,
+//   where both are +-1/2 and map to 2^63 (t = +1/2 is folded to -1/2 so that
+//   T = t 2^64 stays below 2^63);  T = t 2^64 is exact;  round half away
+//   from zero (a no-op once |T| >= 2^52);  then the integer T, |T| <= 2^63,
+//   is split exactly into floor(T / 2^32) and the remainder and converted
+//   to 32-bit halves.
 __device__ __forceinline__ uint64_t from_torus(double x) {
-  const uint64_t bits = (uint64_t)__double_as_longlong(x);
-  const uint32_t ex = (uint32_t)(bits >> 52) & 0x7ffu;
-  if (ex >= 1011u) {
-    const uint64_t m = (bits & 0x000fffffffffffffull) | 0x0010000000000000ull;
-    const uint32_t sh = ex - 1011u;
-    const uint64_t r = sh >= 64u ? 0ull : (m << sh);
-    return (bits >> 63) ? (0ull - r) : r;
-  }
-  const double v = round(x * 18446744073709551616.0);
-  return (uint64_t)(int64_t)v;
+  double t = x - __builtin_rint(x);
+  t = t == 0.5 ? -0.5 : t;
+  const double T = t * 18446744073709551616.0;
+  const double Tt = __builtin_trunc(T);
+  const double Tr = fabs(T - Tt) >= 0.5 ? Tt + __builtin_copysign(1.0, T) : Tt;
+  const double hi = __builtin_floor(Tr * 2.3283064365386963e-10);
+  const double lo = __fma_rn(hi, -4294967296.0, Tr);
+  return ((uint64_t)(uint32_t)(int32_t)hi << 32) | (uint64_t)(uint32_t)lo;
 }
 
 }  // namespace cfft_dev

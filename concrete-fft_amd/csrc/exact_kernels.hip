@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(512) k_exact_fwd(const uint64_t *__restrict__ 
   if (active && li == 0 && lt == 0 && margin) margin[p] = 0.0;
   auto load = [&](uint32_t j) -> c64 {
     if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j], mk((double)limb16(src[j], lb), (double)limb16(src[j + n], lb)));
+    return cmul(twist[j], mk((double)(int32_t)limb16(src[j], lb), (double)(int32_t)limb16(src[j + n], lb)));
   };
   c64 *out = F + t * n;
   auto store = [&](uint32_t pos, c64 v) {

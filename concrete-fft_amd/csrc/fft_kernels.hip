@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(512) k_polymul_generic(
   };
   auto load_b = [&](uint32_t j) -> c64 {
     if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j], mk((double)bb[j], (double)bb[j + n]));
+    return cmul(twist[j], mk(i64_to_f64(bb[j]), i64_to_f64(bb[j + n])));
   };
   auto load_lds = [&](uint32_t j) -> c64 { return lds[lds_pad(j)]; };
   auto store_lds = [&](uint32_t j, c64 v) { lds[lds_pad(j)] = v; };

@@ -49,4 +49,12 @@ hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, u
 hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
                        uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s);
 
+
+// Register-resident kernels for Dif8 plans with base_n == n == 1024 (r8_exec.hpp).
+bool r8_supported(uint32_t log2n);
+hipError_t r8_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
+                        hipStream_t s);
+hipError_t r8_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                      const dc64 *tw, const dc64 *twist, hipStream_t s);
+
 }  // namespace cfft_launch

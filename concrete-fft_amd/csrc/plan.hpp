@@ -33,6 +33,9 @@ struct cfft_plan_s {
   // r16 (register-resident Dif16) twiddle block: T[k*l] as [k][l] (16 x n/16)
   // then T[16 k p] as [k][p] (16 x n/256)
   cfft_launch::dc64 *d_tw_r16 = nullptr;
+  // r8 (register-resident Dif8, n = 1024) twiddle block: T[k*l] as [k][l]
+  // (8 x 128), T[8 k p] as [k][p] (8 x 16), T[64 k p] as [k][p] (8 x 2)
+  cfft_launch::dc64 *d_tw_r8 = nullptr;
   // negacyclic product tables (N = 2n)
   cfft_launch::dc64 *d_twist = nullptr, *d_untwist = nullptr;
 

@@ -5,6 +5,7 @@
 
 #include "launch.hpp"
 #include "r16_exec.hpp"
+#include "r8_exec.hpp"
 
 #include <cstdlib>
 
@@ -60,7 +61,7 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
       const uint32_t j = l + S::T * m;
       const c64 z = twist_i[j];
       fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
-      v[m] = cmul(z, mk((double)b[j], (double)b[j + n]));
+      v[m] = cmul(z, mk(i64_to_f64(b[j]), i64_to_f64(b[j + n])));
     }
     r16_transform<LOG2N, true>(fa, lds, l, tw);
     sched_fence();
@@ -88,6 +89,7 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
 constexpr uint32_t kPW = 8;  // waves per workgroup
 constexpr uint32_t kTwWords = 16 * 64 + 16 * 4;
 constexpr size_t kPersistLds = (kTwWords + kPW * R16<10>::lds_words) * sizeof(c64);
+constexpr bool kPipelined = false;
 
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
@@ -99,7 +101,8 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
   __syncthreads();
-  const uint32_t wave = threadIdx.x >> 6;
+  // uniform per wave: keeps product indices and base addresses in SGPRs
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   c64 *x = smem + kTwWords + wave * S::lds_words;
   const double inv_n = 1.0 / (double)n;
   for (uint64_t poly = (uint64_t)blockIdx.x * kPW + wave; poly < batch;
@@ -112,13 +115,28 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
     uint64_t *c = C + poly * (2 * n);
     const c64 *tws = launder(twist);
     c64 fa[16], v[16];
-    // A, B and the twist are fetched together: one memory latency per product
+    // every load of the product (A, B, twist: 192 VGPRs, nothing else is live
+    // here) is issued before any arithmetic, so the product waits for one
+    // memory latency, not for several register-limited rounds of loads
+    {
+      uint64_t ra[32];
+      int64_t rb[32];
+      c64 tz[16];
 #pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t j = l + S::T * m;
-      const c64 z = tws[j];
-      fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
-      v[m] = cmul(z, mk((double)b[j], (double)b[j + n]));
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = l + S::T * m;
+        ra[2 * m] = a[j];
+        ra[2 * m + 1] = a[j + n];
+        rb[2 * m] = b[j];
+        rb[2 * m + 1] = b[j + n];
+        tz[m] = tws[j];
+      }
+      sched_fence();
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        fa[m] = cmul(tz[m], mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
+        v[m] = cmul(tz[m], mk(i64_to_f64(rb[2 * m]), i64_to_f64(rb[2 * m + 1])));
+      }
     }
     r16w_transform<true, 2>(fa, v, x, l, ltw);
     c64 z[16];
@@ -134,6 +152,82 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
       c[j + n] = from_torus(r.im);
       if ((s & 3) == 3) sched_fence();
     }
+  }
+}
+
+// Software-pipelined variant: the next product's raw A words (64 VGPRs) are
+// fetched before the current product's inverse transform, when only its 64
+// VGPRs of spectrum are live, so their latency overlaps the inverse and the
+// stores; B is fetched at the top of the iteration.
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_r16_polymul_pipelined(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+                        uint64_t *__restrict__ C, uint64_t batch,
+                        const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
+  using S = R16<10>;
+  constexpr uint32_t n = S::n;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
+  __syncthreads();
+  // uniform per wave: keeps product indices and base addresses in SGPRs
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  c64 *x = smem + kTwWords + wave * S::lds_words;
+  const double inv_n = 1.0 / (double)n;
+  const uint64_t step = (uint64_t)gridDim.x * kPW;
+  uint64_t poly = (uint64_t)blockIdx.x * kPW + wave;
+  if (poly >= batch) return;
+  uint64_t ra[32];
+  {
+    const uint32_t l = threadIdx.x & 63u;
+    const uint64_t *a = A + poly * (2 * n);
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      ra[2 * m] = a[j];
+      ra[2 * m + 1] = a[j + n];
+    }
+  }
+  for (;;) {
+    const uint32_t l = launder_u32(threadIdx.x) & 63u;
+    const c64 *tws = launder(twist);
+    const int64_t *b = B + poly * (2 * n);
+    c64 fa[16], v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      const c64 z = tws[j];
+      fa[m] = cmul(z, mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
+      v[m] = cmul(z, mk(i64_to_f64(b[j]), i64_to_f64(b[j + n])));
+    }
+    r16w_transform<true, 2>(fa, v, x, l, ltw);
+    c64 z[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
+    r16_end_to_input<10>(fa, z);
+    // unconditional (clamped) so the old raw words are dead after the twist
+    const uint64_t next = poly + step;
+    {
+      const uint64_t *a = A + (next < batch ? next : poly) * (2 * n);
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = l + S::T * m;
+        ra[2 * m] = a[j];
+        ra[2 * m + 1] = a[j + n];
+      }
+    }
+    sched_fence();
+    r16w_transform<false, 1>(z, z, x, l, ltw);
+    uint64_t *c = C + poly * (2 * n);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
+      c[j] = from_torus(r.re);
+      c[j + n] = from_torus(r.im);
+      if ((s & 3) == 3) sched_fence();
+    }
+    if (next >= batch) break;
+    poly = next;
   }
 }
 
@@ -167,13 +261,22 @@ hipError_t launch_polymul_persistent(const uint64_t *a, const int64_t *b, uint64
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_persistent),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)kPersistLds);
+    if (e == hipSuccess)
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_pipelined),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPersistLds);
     if (e != hipSuccess) return e;
     lds_ok = true;
   }
   const uint64_t want = (batch + kPW - 1) / kPW;
   const uint32_t grid = (uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus);
-  hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
-                     c, batch, reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
+  if (kPipelined)
+    hipLaunchKernelGGL(k_r16_polymul_pipelined, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
+                       c, batch, reinterpret_cast<const c64 *>(tw),
+                       reinterpret_cast<const c64 *>(twist));
+  else
+    hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a,
+                       b, c, batch, reinterpret_cast<const c64 *>(tw),
+                       reinterpret_cast<const c64 *>(twist));
   return hipGetLastError();
 }
 
@@ -190,11 +293,103 @@ hipError_t launch_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint
   return hipGetLastError();
 }
 
+// ---- Dif8, n = 1024: one transform / product per 128-thread workgroup ----
+
+template <bool FWD>
+__global__ void __launch_bounds__(R8::T) k_r8_transform(c64 *__restrict__ data, uint64_t batch,
+                                                        uint64_t stride,
+                                                        const c64 *__restrict__ tw) {
+  __shared__ c64 x[R8::lds_words];
+  const uint32_t l = threadIdx.x;
+  const uint64_t t = blockIdx.x;
+  if (t >= batch) return;
+  c64 *d = data + t * stride;
+  c64 v[8];
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) v[m] = d[l + R8::T * m];
+  r8_transform<FWD>(v, x, l, tw);
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) d[l + R8::T * m] = v[m];
+}
+
+__global__ void __launch_bounds__(R8::T)
+k_r8_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+             uint64_t *__restrict__ C, uint64_t batch, const c64 *__restrict__ tw,
+             const c64 *__restrict__ twist) {
+  constexpr uint32_t n = R8::n;
+  __shared__ c64 x[R8::lds_words];
+  const uint32_t l = threadIdx.x;
+  const uint64_t poly = blockIdx.x;
+  if (poly >= batch) return;
+  const uint64_t *a = A + poly * (2 * n);
+  const int64_t *b = B + poly * (2 * n);
+  uint64_t *c = C + poly * (2 * n);
+  // A is loaded and transformed first; B's raw words are fetched before that
+  // transform (their latency overlaps it) and twisted after it, which keeps
+  // the live state under 128 VGPRs
+  c64 fa[8], fb[8];
+  int64_t rb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) {
+    const uint32_t j = l + R8::T * m;
+    const c64 z = twist[j];
+    fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+  }
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) {
+    const uint32_t j = l + R8::T * m;
+    rb[2 * m] = b[j];
+    rb[2 * m + 1] = b[j + n];
+  }
+  sched_fence();
+  r8_transform<true>(fa, x, l, tw);
+  const c64 *tw1 = launder(twist);
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m)
+    fb[m] = cmul(tw1[l + R8::T * m], mk(i64_to_f64(rb[2 * m]), i64_to_f64(rb[2 * m + 1])));
+  r8_transform<true>(fb, x, l, tw);
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) fa[m] = cmul(fa[m], fb[m]);
+  r8_transform<false>(fa, x, l, tw);
+  const double inv_n = 1.0 / (double)n;
+  const c64 *tws = launder(twist);
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) {
+    const uint32_t j = l + R8::T * m;
+    const c64 r = cmul(untwist_from(tws[j], inv_n), fa[m]);
+    c[j] = from_torus(r.re);
+    c[j + n] = from_torus(r.im);
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
 
 bool r16_supported(uint32_t log2n) { return log2n >= 9 && log2n <= 12; }
+bool r8_supported(uint32_t log2n) { return log2n == 10; }
+
+hipError_t r8_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
+                        hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  if (fwd)
+    hipLaunchKernelGGL(k_r8_transform<true>, dim3((uint32_t)batch), dim3(R8::T), 0, s, d, batch,
+                       stride, t);
+  else
+    hipLaunchKernelGGL(k_r8_transform<false>, dim3((uint32_t)batch), dim3(R8::T), 0, s, d, batch,
+                       stride, t);
+  return hipGetLastError();
+}
+
+hipError_t r8_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                      const dc64 *tw, const dc64 *twist, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_r8_polymul, dim3((uint32_t)batch), dim3(R8::T), 0, s, a, b, c, batch,
+                     reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
+  return hipGetLastError();
+}
 
 hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                          const dc64 *tw, hipStream_t s) {
