@@ -224,6 +224,28 @@ __device__ __forceinline__ constexpr int bitrev_digit(int k) {
   else return ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1);
 }
 
+// ---- streamed operands ----
+// Polynomials are read and written once per product: nontemporal accesses
+// keep them from evicting the twiddle and twist tables every product reuses
+// (measured: +8 % on the N = 2048 batched product).
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_stream(T v, T *p) {
+  __builtin_nontemporal_store(v, p);
+}
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ c64 ld_stream(const c64 *p) {
+  const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p));
+  return mk(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(c64 v, c64 *p) {
+  const dvec2 w = {v.re, v.im};
+  __builtin_nontemporal_store(w, reinterpret_cast<dvec2 *>(p));
+}
+
 // ---- integer and torus conversions (caller-side convention) ----
 // (double)v for any int64 v, correctly rounded: hi 2^32 + lo is formed
 // exactly inside one fma and rounded once.

@@ -99,8 +99,9 @@ __global__ void __launch_bounds__(512) k_extprod(const c64 *__restrict__ keyf,
     const uint64_t *src = in + (uint64_t)poly * 2ull * n;
     auto load = [&](uint32_t j) -> c64 {
       if (!active) return mk(0.0, 0.0);
-      return cmul(twist[j], mk((double)(int32_t)gadget_digit(src[j], lvl, levels, base_log),
-                               (double)(int32_t)gadget_digit(src[j + n], lvl, levels, base_log)));
+      return cmul(twist[j],
+                  mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
+                     (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
     };
     const c64 *krow = keyf + (uint64_t)r * k1 * n;
     auto store = [&](uint32_t pos, c64 v) {
@@ -119,8 +120,8 @@ __global__ void __launch_bounds__(512) k_extprod(const c64 *__restrict__ keyf,
     auto store = [&](uint32_t j, c64 v) {
       if (!active) return;
       const c64 r = cmul(untwist[j], v);
-      o[j] = from_torus(r.re);
-      o[j + n] = from_torus(r.im);
+      st_stream(from_torus(r.re), &o[j]);
+      st_stream(from_torus(r.im), &o[j + n]);
     };
     run_program<E>(inv, lds, lt, T, tw, load, store);
   }

@@ -34,9 +34,9 @@ __global__ void __launch_bounds__(512) k_fft_lds(c64 *__restrict__ data, uint64_
   c64 *lds = smem + g * lds_words(n);
   const uint64_t tb = tr >> chunks_log2, tc = tr & ((1ull << chunks_log2) - 1);
   c64 *gp = data + tb * stride + (tc << log2n);
-  auto load = [&](uint32_t pos) -> c64 { return active ? gp[pos] : mk(0.0, 0.0); };
+  auto load = [&](uint32_t pos) -> c64 { return active ? ld_stream(&gp[pos]) : mk(0.0, 0.0); };
   auto store = [&](uint32_t pos, c64 v) {
-    if (active) gp[pos] = v;
+    if (active) st_stream(v, &gp[pos]);
   };
   run_program<E>(prog, lds, lt, T, tw, load, store);
 }

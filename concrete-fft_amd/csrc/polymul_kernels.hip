@@ -32,10 +32,10 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) k_r16_transform(c64 *__restrict
   c64 *x = data + t * stride;
   c64 v[16];
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) v[m] = x[l + S::T * m];
+  for (uint32_t m = 0; m < 16; ++m) v[m] = ld_stream(&x[l + S::T * m]);
   r16_transform<LOG2N, FWD>(v, lds, l, tw);
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) x[l + S::T * S::end_m(s)] = v[s];
+  for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &x[l + S::T * S::end_m(s)]);
 }
 
 template <int LOG2N>
@@ -60,8 +60,8 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
     for (uint32_t m = 0; m < 16; ++m) {
       const uint32_t j = l + S::T * m;
       const c64 z = twist_i[j];
-      fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
-      v[m] = cmul(z, mk(i64_to_f64(b[j]), i64_to_f64(b[j + n])));
+      fa[m] = cmul(z, mk(torus_to_f64(ld_stream(&a[j])), torus_to_f64(ld_stream(&a[j + n]))));
+      v[m] = cmul(z, mk(i64_to_f64(ld_stream(&b[j])), i64_to_f64(ld_stream(&b[j + n]))));
     }
     r16_transform<LOG2N, true>(fa, lds, l, tw);
     sched_fence();
@@ -76,8 +76,8 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
     for (uint32_t s = 0; s < 16; ++s) {
       const uint32_t j = l + S::T * S::end_m(s);
       const c64 r = cmul(untwist_from(twist_i[j], inv_n), z[s]);
-      c[j] = from_torus(r.re);
-      c[j + n] = from_torus(r.im);
+      st_stream(from_torus(r.re), &c[j]);
+      st_stream(from_torus(r.im), &c[j + n]);
       if ((s & 3) == 3) sched_fence();
     }
   }
@@ -89,7 +89,6 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per
 constexpr uint32_t kPW = 8;  // waves per workgroup
 constexpr uint32_t kTwWords = 16 * 64 + 16 * 4;
 constexpr size_t kPersistLds = (kTwWords + kPW * R16<10>::lds_words) * sizeof(c64);
-constexpr bool kPipelined = false;
 
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
@@ -125,10 +124,10 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
 #pragma unroll
       for (uint32_t m = 0; m < 16; ++m) {
         const uint32_t j = l + S::T * m;
-        ra[2 * m] = a[j];
-        ra[2 * m + 1] = a[j + n];
-        rb[2 * m] = b[j];
-        rb[2 * m + 1] = b[j + n];
+        ra[2 * m] = ld_stream(&a[j]);
+        ra[2 * m + 1] = ld_stream(&a[j + n]);
+        rb[2 * m] = ld_stream(&b[j]);
+        rb[2 * m + 1] = ld_stream(&b[j + n]);
         tz[m] = tws[j];
       }
       sched_fence();
@@ -148,86 +147,10 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
     for (uint32_t s = 0; s < 16; ++s) {
       const uint32_t j = l + S::T * S::end_m(s);
       const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
-      c[j] = from_torus(r.re);
-      c[j + n] = from_torus(r.im);
+      st_stream(from_torus(r.re), &c[j]);
+      st_stream(from_torus(r.im), &c[j + n]);
       if ((s & 3) == 3) sched_fence();
     }
-  }
-}
-
-// Software-pipelined variant: the next product's raw A words (64 VGPRs) are
-// fetched before the current product's inverse transform, when only its 64
-// VGPRs of spectrum are live, so their latency overlaps the inverse and the
-// stores; B is fetched at the top of the iteration.
-__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_r16_polymul_pipelined(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
-                        uint64_t *__restrict__ C, uint64_t batch,
-                        const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
-  using S = R16<10>;
-  constexpr uint32_t n = S::n;
-  extern __shared__ c64 smem[];
-  c64 *ltw = smem;
-  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
-  __syncthreads();
-  // uniform per wave: keeps product indices and base addresses in SGPRs
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  c64 *x = smem + kTwWords + wave * S::lds_words;
-  const double inv_n = 1.0 / (double)n;
-  const uint64_t step = (uint64_t)gridDim.x * kPW;
-  uint64_t poly = (uint64_t)blockIdx.x * kPW + wave;
-  if (poly >= batch) return;
-  uint64_t ra[32];
-  {
-    const uint32_t l = threadIdx.x & 63u;
-    const uint64_t *a = A + poly * (2 * n);
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t j = l + S::T * m;
-      ra[2 * m] = a[j];
-      ra[2 * m + 1] = a[j + n];
-    }
-  }
-  for (;;) {
-    const uint32_t l = launder_u32(threadIdx.x) & 63u;
-    const c64 *tws = launder(twist);
-    const int64_t *b = B + poly * (2 * n);
-    c64 fa[16], v[16];
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t j = l + S::T * m;
-      const c64 z = tws[j];
-      fa[m] = cmul(z, mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
-      v[m] = cmul(z, mk(i64_to_f64(b[j]), i64_to_f64(b[j + n])));
-    }
-    r16w_transform<true, 2>(fa, v, x, l, ltw);
-    c64 z[16];
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
-    r16_end_to_input<10>(fa, z);
-    // unconditional (clamped) so the old raw words are dead after the twist
-    const uint64_t next = poly + step;
-    {
-      const uint64_t *a = A + (next < batch ? next : poly) * (2 * n);
-#pragma unroll
-      for (uint32_t m = 0; m < 16; ++m) {
-        const uint32_t j = l + S::T * m;
-        ra[2 * m] = a[j];
-        ra[2 * m + 1] = a[j + n];
-      }
-    }
-    sched_fence();
-    r16w_transform<false, 1>(z, z, x, l, ltw);
-    uint64_t *c = C + poly * (2 * n);
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-      const uint32_t j = l + S::T * S::end_m(s);
-      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
-      c[j] = from_torus(r.re);
-      c[j + n] = from_torus(r.im);
-      if ((s & 3) == 3) sched_fence();
-    }
-    if (next >= batch) break;
-    poly = next;
   }
 }
 
@@ -261,22 +184,13 @@ hipError_t launch_polymul_persistent(const uint64_t *a, const int64_t *b, uint64
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_persistent),
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)kPersistLds);
-    if (e == hipSuccess)
-      e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_pipelined),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPersistLds);
     if (e != hipSuccess) return e;
     lds_ok = true;
   }
   const uint64_t want = (batch + kPW - 1) / kPW;
   const uint32_t grid = (uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus);
-  if (kPipelined)
-    hipLaunchKernelGGL(k_r16_polymul_pipelined, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
-                       c, batch, reinterpret_cast<const c64 *>(tw),
-                       reinterpret_cast<const c64 *>(twist));
-  else
-    hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a,
-                       b, c, batch, reinterpret_cast<const c64 *>(tw),
-                       reinterpret_cast<const c64 *>(twist));
+  hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
+                     c, batch, reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
   return hipGetLastError();
 }
 
@@ -306,10 +220,10 @@ __global__ void __launch_bounds__(R8::T) k_r8_transform(c64 *__restrict__ data, 
   c64 *d = data + t * stride;
   c64 v[8];
 #pragma unroll
-  for (uint32_t m = 0; m < 8; ++m) v[m] = d[l + R8::T * m];
+  for (uint32_t m = 0; m < 8; ++m) v[m] = ld_stream(&d[l + R8::T * m]);
   r8_transform<FWD>(v, x, l, tw);
 #pragma unroll
-  for (uint32_t m = 0; m < 8; ++m) d[l + R8::T * m] = v[m];
+  for (uint32_t m = 0; m < 8; ++m) st_stream(v[m], &d[l + R8::T * m]);
 }
 
 __global__ void __launch_bounds__(R8::T)
@@ -333,13 +247,13 @@ k_r8_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   for (uint32_t m = 0; m < 8; ++m) {
     const uint32_t j = l + R8::T * m;
     const c64 z = twist[j];
-    fa[m] = cmul(z, mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+    fa[m] = cmul(z, mk(torus_to_f64(ld_stream(&a[j])), torus_to_f64(ld_stream(&a[j + n]))));
   }
 #pragma unroll
   for (uint32_t m = 0; m < 8; ++m) {
     const uint32_t j = l + R8::T * m;
-    rb[2 * m] = b[j];
-    rb[2 * m + 1] = b[j + n];
+    rb[2 * m] = ld_stream(&b[j]);
+    rb[2 * m + 1] = ld_stream(&b[j + n]);
   }
   sched_fence();
   r8_transform<true>(fa, x, l, tw);
@@ -357,8 +271,8 @@ k_r8_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   for (uint32_t m = 0; m < 8; ++m) {
     const uint32_t j = l + R8::T * m;
     const c64 r = cmul(untwist_from(tws[j], inv_n), fa[m]);
-    c[j] = from_torus(r.re);
-    c[j + n] = from_torus(r.im);
+    st_stream(from_torus(r.re), &c[j]);
+    st_stream(from_torus(r.im), &c[j + n]);
   }
 }
 

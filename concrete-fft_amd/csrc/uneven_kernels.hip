@@ -65,10 +65,10 @@ __device__ __forceinline__ void tile_generic(const UClass &C, const UTile &tile,
   (void)G;
   auto load_x = [&](uint32_t j) -> c64 {
     if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j], mk(P.x[j], P.x[j + n]));
+    return cmul(twist[j], mk(ld_stream(&P.x[j]), ld_stream(&P.x[j + n])));
   };
   auto store_f = [&](uint32_t pos, c64 v) {
-    if (active) P.f[pos] = v;
+    if (active) st_stream(v, &P.f[pos]);
     lds[lds_pad(pos)] = v;
   };
   run_program<E>(C.fwd, lds, lt, T, tw, load_x, store_f);
@@ -76,8 +76,8 @@ __device__ __forceinline__ void tile_generic(const UClass &C, const UTile &tile,
   auto store_y = [&](uint32_t j, c64 v) {
     if (!active) return;
     const c64 r = cmul(untwist[j], v);
-    P.y[j] = r.re;
-    P.y[j + n] = r.im;
+    st_stream(r.re, &P.y[j]);
+    st_stream(r.im, &P.y[j + n]);
   };
   run_program<E>(C.inv, lds, lt, T, tw, load_f, store_y);
 }
