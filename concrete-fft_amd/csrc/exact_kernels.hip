@@ -115,9 +115,15 @@ __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
     };
     run_program<E>(inv, lds, lt, T, tw, load, store);
   }
-  if (active && margin)
-    atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
-              (unsigned long long)__double_as_longlong(worst));
+  // one atomic per transform group per wave: max over the group's lanes first
+  // (non-negative doubles order like their bit patterns)
+  if (margin) {
+    const uint32_t seg = T < 64 ? T : 64;
+    for (uint32_t off = 1; off < seg; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
+    if (active && (threadIdx.x & (seg - 1)) == 0)
+      atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
+                (unsigned long long)__double_as_longlong(worst));
+  }
 }
 
 hipError_t set_lds(const void *kernel, size_t bytes) {

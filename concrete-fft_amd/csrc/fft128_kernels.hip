@@ -390,24 +390,33 @@ __global__ void __launch_bounds__(256) k_exact128_combine(const double *__restri
                                                           double *margin, uint64_t batch,
                                                           uint32_t L) {
   const uint32_t n = 1u << L;
+  const uint64_t total = batch * n;
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= batch * n) return;
-  const uint64_t p = gid >> L;
-  const uint32_t j = (uint32_t)(gid & (n - 1));
+  const bool active = gid < total;
+  const uint64_t g = active ? gid : total - 1;
+  const uint64_t p = g >> L;
+  const uint32_t j = (uint32_t)(g & (n - 1));
   const uint64_t plane = 4 * batch * (uint64_t)n;
   const double *re0 = S, *re1 = S + plane, *im0 = S + 2 * plane, *im1 = S + 3 * plane;
-  const uint64_t i1 = batch * n + gid;
+  const uint64_t i1 = batch * n + g;
   double d0, d1, d2, d3;
-  const uint64_t c_re = dd_round_mod64(re0[gid], re1[gid], d0) +
+  const uint64_t c_re = dd_round_mod64(re0[g], re1[g], d0) +
                         (dd_round_mod64(re0[i1], re1[i1], d1) << 32);
-  const uint64_t c_im = dd_round_mod64(im0[gid], im1[gid], d2) +
+  const uint64_t c_im = dd_round_mod64(im0[g], im1[g], d2) +
                         (dd_round_mod64(im0[i1], im1[i1], d3) << 32);
-  C[p * 2 * n + j] = c_re;
-  C[p * 2 * n + j + n] = c_im;
+  if (active) {
+    C[p * 2 * n + j] = c_re;
+    C[p * 2 * n + j + n] = c_im;
+  }
   if (margin) {
-    const double w = fmax(fmax(d0, d1), fmax(d2, d3));
-    atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
-              (unsigned long long)__double_as_longlong(w));
+    // one atomic per product per wave: max over the product's lanes first
+    // (non-negative doubles order like their bit patterns)
+    double w = active ? fmax(fmax(d0, d1), fmax(d2, d3)) : 0.0;
+    const uint32_t seg = n < 64 ? n : 64;
+    for (uint32_t off = 1; off < seg; off <<= 1) w = fmax(w, __shfl_xor(w, (int)off));
+    if (active && (threadIdx.x & (seg - 1)) == 0)
+      atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
+                (unsigned long long)__double_as_longlong(w));
   }
 }
 
