@@ -127,6 +127,78 @@ __global__ void __launch_bounds__(512) k_extprod(const c64 *__restrict__ keyf,
   }
 }
 
+// Split form for small batches (latency): one workgroup per (GLWE, row)
+// transforms its digit polynomial into a Fourier scratch S[(p rows + r) n];
+// then one workgroup per (GLWE, column) accumulates sum_r key[r][c] * S[p, r]
+// in the same row order (same bits as the fused kernel), inverse, torus.
+template <int E>
+__global__ void __launch_bounds__(512) k_extprod_rows(const uint64_t *__restrict__ glwe,
+                                                      c64 *__restrict__ S, uint64_t nblocks,
+                                                      uint32_t levels, uint32_t base_log,
+                                                      const c64 *__restrict__ tw,
+                                                      const c64 *__restrict__ twist,
+                                                      Program fwd) {
+  extern __shared__ c64 smem[];
+  const uint32_t n = 1u << fwd.log2n;
+  const uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t t = (uint64_t)blockIdx.x * G + g;  // (p * k1 + poly) * levels + lvl
+  const bool active = t < nblocks;
+  c64 *lds = smem + g * lds_words(n);
+  const uint32_t lvl = (uint32_t)(t % levels);
+  const uint64_t *src = glwe + (t / levels) * 2ull * n;
+  auto load = [&](uint32_t j) -> c64 {
+    if (!active) return mk(0.0, 0.0);
+    return cmul(twist[j],
+                mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
+                   (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
+  };
+  c64 *dst = S + t * n;
+  auto store = [&](uint32_t pos, c64 v) {
+    if (active) dst[pos] = v;
+  };
+  run_program<E>(fwd, lds, lt, T, tw, load, store);
+}
+
+template <int E>
+__global__ void __launch_bounds__(512) k_extprod_cols(const c64 *__restrict__ keyf,
+                                                      const c64 *__restrict__ S,
+                                                      uint64_t *__restrict__ out,
+                                                      uint64_t nblocks, uint32_t k1,
+                                                      uint32_t rows, const c64 *__restrict__ tw,
+                                                      const c64 *__restrict__ untwist,
+                                                      Program inv) {
+  extern __shared__ c64 smem[];
+  const uint32_t n = 1u << inv.log2n;
+  const uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t t = (uint64_t)blockIdx.x * G + g;  // p * k1 + c
+  const bool active = t < nblocks;
+  c64 *lds = smem + g * lds_words(n);
+  const uint64_t p = t / k1;
+  const uint32_t c = (uint32_t)(t % k1);
+  const c64 *sp = S + p * (uint64_t)rows * n;
+  auto load = [&](uint32_t pos) -> c64 {
+    if (!active) return mk(0.0, 0.0);
+    c64 acc = mk(0.0, 0.0);
+    for (uint32_t r = 0; r < rows; ++r) {
+      const c64 pr = cmul(keyf[((uint64_t)r * k1 + c) * n + pos], sp[(uint64_t)r * n + pos]);
+      acc = mk(acc.re + pr.re, acc.im + pr.im);
+    }
+    return acc;
+  };
+  uint64_t *o = out + t * 2ull * n;
+  auto store = [&](uint32_t j, c64 v) {
+    if (!active) return;
+    const c64 r = cmul(untwist[j], v);
+    st_stream(from_torus(r.re), &o[j]);
+    st_stream(from_torus(r.im), &o[j + n]);
+  };
+  run_program<E>(inv, lds, lt, T, tw, load, store);
+}
+
 hipError_t set_lds(const void *kernel, size_t bytes) {
   static std::mutex mu;
   static std::set<const void *> done;
@@ -190,6 +262,28 @@ hipError_t launch_ext(const Program &fwd, const Program &inv, const c64 *keyf,
   return hipGetLastError();
 }
 
+template <int E>
+hipError_t launch_split(const Program &fwd, const Program &inv, const c64 *keyf,
+                        const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
+                        uint32_t levels, uint32_t base_log, const c64 *tw, const c64 *twist,
+                        const c64 *untwist, c64 *S, hipStream_t s) {
+  const uint32_t words = lds_words(1u << fwd.log2n);
+  const Geom g = geom(fwd.log2n, 64 * 1024, words);
+  const size_t lds = (size_t)g.G * words * sizeof(c64);
+  hipError_t e = set_lds(reinterpret_cast<const void *>(k_extprod_rows<E>), lds);
+  if (e == hipSuccess) e = set_lds(reinterpret_cast<const void *>(k_extprod_cols<E>), lds);
+  if (e != hipSuccess) return e;
+  const uint64_t nrows = batch * k1 * levels, ncols = batch * k1;
+  hipLaunchKernelGGL(k_extprod_rows<E>, dim3((uint32_t)((nrows + g.G - 1) / g.G)),
+                     dim3(g.threads), lds, s, glwe, S, nrows, levels, base_log, tw, twist, fwd);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_extprod_cols<E>, dim3((uint32_t)((ncols + g.G - 1) / g.G)),
+                     dim3(g.threads), lds, s, keyf, S, out, ncols, k1, k1 * levels, tw, untwist,
+                     inv);
+  return hipGetLastError();
+}
+
 int pick_e(uint32_t log2n) {
   const uint32_t n = 1u << log2n;
   return n >= 16 ? 16 : (int)n;
@@ -230,6 +324,25 @@ hipError_t external_product(const Program &fwd, const Program &inv, const dc64 *
     case 8: return launch_ext<8>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
     default:
       return launch_ext<16>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
+  }
+}
+
+hipError_t external_product_split(const Program &fwd, const Program &inv, const dc64 *keyf,
+                                  const uint64_t *glwe, uint64_t *out, uint64_t batch,
+                                  uint32_t k1, uint32_t levels, uint32_t base_log,
+                                  const dc64 *tw, const dc64 *twist, const dc64 *untwist,
+                                  dc64 *scratch, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const c64 *kf = reinterpret_cast<const c64 *>(keyf);
+  const c64 *t = reinterpret_cast<const c64 *>(tw), *z = reinterpret_cast<const c64 *>(twist);
+  const c64 *u = reinterpret_cast<const c64 *>(untwist);
+  c64 *S = reinterpret_cast<c64 *>(scratch);
+  switch (pick_e(fwd.log2n)) {
+    case 2: return launch_split<2>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, S, s);
+    case 4: return launch_split<4>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, S, s);
+    case 8: return launch_split<8>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, S, s);
+    default:
+      return launch_split<16>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, S, s);
   }
 }
 

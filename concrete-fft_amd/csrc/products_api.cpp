@@ -22,9 +22,17 @@ hipError_t external_product(const cfft_prog::Program &fwd, const cfft_prog::Prog
                             uint32_t k1, uint32_t levels, uint32_t base_log, const dc64 *tw,
                             const dc64 *twist, const dc64 *untwist, hipStream_t s);
 size_t external_product_lds_bytes(uint32_t log2n, uint32_t k1);
+hipError_t external_product_split(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                                  const dc64 *keyf, const uint64_t *glwe, uint64_t *out,
+                                  uint64_t batch, uint32_t k1, uint32_t levels,
+                                  uint32_t base_log, const dc64 *tw, const dc64 *twist,
+                                  const dc64 *untwist, dc64 *scratch, hipStream_t s);
 }
 
 namespace {
+// batch * k1 * levels at or below which the external product runs split
+constexpr uint64_t kSplitRowBlocks = 1024;
+
 int check_extprod_plan(cfft_plan plan) {
   if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
   if (plan->kind != cfft_plan_s::UNORDERED)
@@ -98,6 +106,25 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
                                       "not fit in one workgroup's LDS");
   if (batch == 0) return CFFT_SUCCESS;
   if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  // Small batches: one workgroup per GLWE leaves most of the device idle and
+  // runs k1 * levels + k1 transforms back to back; the split form spreads the
+  // rows and columns over workgroups (through a Fourier scratch) instead.
+  // Same accumulation order, same bits.
+  const uint64_t row_blocks = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
+  if (row_blocks <= kSplitRowBlocks) {
+    hipStream_t s = (hipStream_t)stream;
+    const size_t bytes = row_blocks * plan->n * sizeof(cfft_launch::dc64);
+    void *S = nullptr;
+    e = hipMallocAsync(&S, bytes, s);
+    if (e != hipSuccess) return hip_error(e, "hipMallocAsync (external product scratch)");
+    e = cfft_launch::external_product_split(
+        plan->fwd_full, plan->inv_full, reinterpret_cast<const cfft_launch::dc64 *>(key_fourier),
+        glwe, out, batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
+        plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+    hipError_t e2 = hipFreeAsync(S, s);
+    if (e != hipSuccess) return hip_error(e, "external product (split)");
+    return hip_error(e2, "hipFreeAsync");
+  }
   return hip_error(cfft_launch::external_product(
                        plan->fwd_full, plan->inv_full,
                        reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,

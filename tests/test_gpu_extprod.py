@@ -101,3 +101,23 @@ def test_external_product_errors(cfft):
     with pytest.raises(CfftPanic, match="unordered"):
         products.external_product(Plan(1024, Method.UserProvided(FftAlgo.Dif4)), kf, 2, 3, 8, g,
                                   o, 1)
+
+
+@pytest.mark.parametrize("N", [1024, 4096])
+def test_fused_and_split_forms_agree(cfft, oracle, N):
+    # batch * k1 * levels above 1024 runs the fused one-workgroup-per-GLWE
+    # kernel; smaller calls run the split row / column kernels
+    k1, levels, base_log = 2, 3, 8
+    rng = np.random.default_rng(N + 99)
+    key = rng.integers(0, M, (k1 * levels, k1, N), dtype=np.uint64)
+    glwe = rng.integers(0, M, (200, k1, N), dtype=np.uint64)
+    plan = _plan(N, 6, min(N // 2, 1024))
+    kf = _key_fourier(plan, key, k1, levels)
+    fused = _extprod(plan, kf, k1, levels, base_log, glwe)
+    split = np.concatenate([_extprod(plan, kf, k1, levels, base_log, glwe[i:i + 100])
+                            for i in (0, 100)])
+    assert np.array_equal(fused, split)
+    kfh = oracle.torus_to_fourier(key, 6, min(N // 2, 1024))
+    for b in (0, 199):
+        assert np.array_equal(fused[b], oracle.external_product(N, k1, levels, base_log, kfh, 6,
+                                                                min(N // 2, 1024), glwe[b]))
