@@ -20,6 +20,7 @@
 #include <set>
 
 #include "launch.hpp"
+#include "r16_exec.hpp"
 #include "stockham_exec.hpp"
 
 using namespace cfft_dev;
@@ -72,6 +73,11 @@ __global__ void __launch_bounds__(512) k_exact_fwd(const uint64_t *__restrict__ 
   run_program<E>(fwd, lds, lt, T, tw, load, store);
 }
 
+// One block per product, one transform group per weight k (4 groups): group
+// k runs the inverse of sum_{i+j=k} F[a_i] F[b_j], rounds, shifts by 16 k
+// and leaves its 2n words in its own LDS region (the last pass's LDS reads
+// are complete by then); after the final barrier all threads sum the four
+// regions mod 2^64 and store C.
 template <int E>
 __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
                                                    uint64_t *__restrict__ C, double *margin,
@@ -80,47 +86,162 @@ __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
   extern __shared__ c64 smem[];
   const uint32_t n = 1u << inv.log2n;
   const uint32_t T = n / E;
-  const uint32_t G = blockDim.x / T;
-  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
-  const uint64_t p = (uint64_t)blockIdx.x * G + g;
-  const bool active = p < batch;
-  c64 *lds = smem + g * lds_words(n);
+  const uint32_t k = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t p = blockIdx.x;
+  const uint32_t words = lds_words(n);
+  c64 *lds = smem + k * words;
+  uint64_t *part = reinterpret_cast<uint64_t *>(lds);  // 2n words fit in `words` points
   const c64 *fp = F + p * 8ull * n;
-  uint64_t *c = C + p * (2ull * n);
   double worst = 0.0;
-  for (uint32_t k = 0; k < 4; ++k) {
-    auto load = [&](uint32_t pos) -> c64 {
-      if (!active) return mk(0.0, 0.0);
-      c64 acc = cmul(fp[0 * n + pos], fp[(4 + k) * (size_t)n + pos]);
-      for (uint32_t i = 1; i <= k; ++i) {
-        const c64 pr = cmul(fp[(size_t)i * n + pos], fp[(size_t)(4 + k - i) * n + pos]);
-        acc = cadd(acc, pr);
-      }
-      return acc;
-    };
-    auto store = [&](uint32_t pos, c64 v) {
-      if (!active) return;
-      const c64 r = cmul(untwist[pos], v);
-      const double x0 = round(r.re), x1 = round(r.im);
-      worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
-      const uint64_t d0 = (uint64_t)(int64_t)x0 << (16 * k);
-      const uint64_t d1 = (uint64_t)(int64_t)x1 << (16 * k);
-      if (k == 0) {
-        c[pos] = d0;
-        c[pos + n] = d1;
-      } else {
-        c[pos] += d0;
-        c[pos + n] += d1;
-      }
-    };
-    run_program<E>(inv, lds, lt, T, tw, load, store);
-  }
+  auto load = [&](uint32_t pos) -> c64 {
+    c64 acc = cmul(fp[0 * n + pos], fp[(4 + k) * (size_t)n + pos]);
+    for (uint32_t i = 1; i <= k; ++i) {
+      const c64 pr = cmul(fp[(size_t)i * n + pos], fp[(size_t)(4 + k - i) * n + pos]);
+      acc = cadd(acc, pr);
+    }
+    return acc;
+  };
+  auto store = [&](uint32_t pos, c64 v) {
+    const c64 r = cmul(untwist[pos], v);
+    const double x0 = round(r.re), x1 = round(r.im);
+    worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
+    part[pos] = (uint64_t)(int64_t)x0 << (16 * k);
+    part[n + pos] = (uint64_t)(int64_t)x1 << (16 * k);
+  };
+  run_program<E>(inv, lds, lt, T, tw, load, store);  // ends with a barrier
+  const uint64_t *p0 = reinterpret_cast<const uint64_t *>(smem);
+  const uint64_t *p1 = reinterpret_cast<const uint64_t *>(smem + words);
+  const uint64_t *p2 = reinterpret_cast<const uint64_t *>(smem + 2 * words);
+  const uint64_t *p3 = reinterpret_cast<const uint64_t *>(smem + 3 * words);
+  uint64_t *c = C + p * (2ull * n);
+  for (uint32_t q = threadIdx.x; q < 2 * n; q += blockDim.x)
+    st_stream(p0[q] + p1[q] + p2[q] + p3[q], &c[q]);
   // one atomic per transform group per wave: max over the group's lanes first
   // (non-negative doubles order like their bit patterns)
   if (margin) {
     const uint32_t seg = T < 64 ? T : 64;
     for (uint32_t off = 1; off < seg; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
-    if (active && (threadIdx.x & (seg - 1)) == 0)
+    if ((threadIdx.x & (seg - 1)) == 0)
+      atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
+                (unsigned long long)__double_as_longlong(worst));
+  }
+}
+
+// ---- register-resident form for Dif16 plans with n = 1024 (N = 2048) ----
+// Forward: one wave per (product, limb index i) transforms a_i and b_i
+// together (wave-local Dif16 schedule, twiddles shared); spectra go to the
+// same scratch layout as above.  8 waves per workgroup share the twiddle
+// block staged in LDS.
+constexpr uint32_t kXW = 8;
+constexpr uint32_t kXTw = 16 * 64 + 16 * 4;
+constexpr size_t kXLds = (kXTw + kXW * R16<10>::lds_words) * sizeof(c64);
+
+__global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
+                c64 *__restrict__ F, double *margin, uint64_t batch,
+                const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
+  using S = R16<10>;
+  constexpr uint32_t n = S::n;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_r16[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t u = (uint64_t)blockIdx.x * kXW + wave;  // 4 p + i
+  if (u >= batch * 4) return;
+  c64 *x = smem + kXTw + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  const uint64_t p = u >> 2;
+  const uint32_t li = (uint32_t)(u & 3);
+  if (li == 0 && l == 0 && margin) margin[p] = 0.0;
+  const uint64_t *a = A + p * (2 * n);
+  const uint64_t *b = B + p * (2 * n);
+  c64 fa[16], fb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    const uint32_t j = l + S::T * m;
+    const c64 z = twist[j];
+    fa[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&a[j]), li),
+                       (double)(int32_t)limb16(ld_stream(&a[j + n]), li)));
+    fb[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&b[j]), li),
+                       (double)(int32_t)limb16(ld_stream(&b[j + n]), li)));
+  }
+  r16w_transform<true, 2>(fa, fb, x, l, ltw);
+  c64 *oa = F + (8 * p + li) * n, *ob = F + (8 * p + 4 + li) * n;
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t pos = l + S::T * S::end_m(s);
+    oa[pos] = fa[s];
+    ob[pos] = fb[s];
+  }
+}
+
+// Inverse: 4 waves per product, wave k computes 2^(16 k) D_k (its weight's
+// Fourier sum, inverse transform, rounding, shift) into its own LDS buffer;
+// after a barrier the product's first wave sums the four buffers mod 2^64.
+__global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *margin,
+                uint64_t batch, const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
+  using S = R16<10>;
+  constexpr uint32_t n = S::n;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_r16[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t p = (uint64_t)blockIdx.x * (kXW / 4) + (wave >> 2);
+  const uint32_t k = wave & 3;
+  const bool active = p < batch;  // every wave reaches the barrier below
+  c64 *x = smem + kXTw + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  const c64 *fp = F + (active ? p : 0) * 8ull * n;
+  const double inv_n = 1.0 / (double)n;
+  double worst = 0.0;
+  c64 v[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    const uint32_t pos = l + S::T * m;
+    c64 d = cmul(fp[pos], fp[(4 + k) * n + pos]);
+    for (uint32_t i = 1; i <= k; ++i)
+      d = cadd(d, cmul(fp[i * n + pos], fp[(4 + k - i) * n + pos]));
+    v[m] = d;
+    if ((m & 3) == 3) sched_fence();  // bounds the loads in flight
+  }
+  r16w_transform<false, 1>(v, v, x, l, ltw);
+  uint64_t part[32];
+  const c64 *tws = launder(twist);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t j = l + S::T * S::end_m(s);
+    const c64 tz = tws[j];
+    const c64 r = cmul(mk(tz.re * inv_n, (-tz.im) * inv_n), v[s]);
+    const double x0 = round(r.re), x1 = round(r.im);
+    worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
+    part[2 * s] = (uint64_t)(int64_t)x0 << (16 * k);
+    part[2 * s + 1] = (uint64_t)(int64_t)x1 << (16 * k);
+  }
+  // the transform's last LDS reads are behind its final wave fence; reuse the
+  // buffer for this wave's 32 words per lane
+  uint64_t *xb = reinterpret_cast<uint64_t *>(x);
+#pragma unroll
+  for (uint32_t q = 0; q < 32; ++q) xb[q * 64 + l] = part[q];
+  __syncthreads();
+  if (active && k == 0) {
+    const uint64_t *b1 = reinterpret_cast<const uint64_t *>(x + 1 * S::lds_words);
+    const uint64_t *b2 = reinterpret_cast<const uint64_t *>(x + 2 * S::lds_words);
+    const uint64_t *b3 = reinterpret_cast<const uint64_t *>(x + 3 * S::lds_words);
+    uint64_t *c = C + p * (2 * n);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const uint32_t q0 = (2 * s) * 64 + l, q1 = (2 * s + 1) * 64 + l;
+      st_stream(part[2 * s] + b1[q0] + b2[q0] + b3[q0], &c[j]);
+      st_stream(part[2 * s + 1] + b1[q1] + b2[q1] + b3[q1], &c[j + n]);
+    }
+  }
+  if (margin) {
+    for (uint32_t off = 1; off < 64; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
+    if (active && l == 0)
       atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
                 (unsigned long long)__double_as_longlong(worst));
   }
@@ -165,16 +286,19 @@ hipError_t launch_e(const Program &fwd, const Program &inv, const uint64_t *a, c
                     uint64_t *c, double *margin, uint64_t batch, const c64 *tw,
                     const c64 *twist, const c64 *untwist, c64 *F, hipStream_t s) {
   const Geom g = geom(fwd.log2n);
+  const uint32_t n = 1u << fwd.log2n;
+  const uint32_t T = n / (uint32_t)E;
+  const size_t inv_lds = 4 * (size_t)lds_words(n) * sizeof(c64);
   hipError_t e = set_lds(reinterpret_cast<const void *>(k_exact_fwd<E>), g.lds);
-  if (e == hipSuccess) e = set_lds(reinterpret_cast<const void *>(k_exact_inv<E>), g.lds);
+  if (e == hipSuccess) e = set_lds(reinterpret_cast<const void *>(k_exact_inv<E>), inv_lds);
   if (e != hipSuccess) return e;
   const uint64_t nt = batch * 8;
   hipLaunchKernelGGL(k_exact_fwd<E>, dim3((uint32_t)((nt + g.G - 1) / g.G)), dim3(g.threads),
                      g.lds, s, a, b, F, margin, batch, tw, twist, fwd);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_exact_inv<E>, dim3((uint32_t)((batch + g.G - 1) / g.G)), dim3(g.threads),
-                     g.lds, s, F, c, margin, batch, tw, untwist, inv);
+  hipLaunchKernelGGL(k_exact_inv<E>, dim3((uint32_t)batch), dim3(4 * T), inv_lds, s, F, c, margin,
+                     batch, tw, untwist, inv);
   return hipGetLastError();
 }
 
@@ -198,6 +322,43 @@ hipError_t exact_polymul(const Program &fwd, const Program &inv, const uint64_t 
     case 8: return launch_e<8>(fwd, inv, a, b, c, margin, batch, t, tws, utw, f, s);
     default: return launch_e<16>(fwd, inv, a, b, c, margin, batch, t, tws, utw, f, s);
   }
+}
+
+// Register-resident form: Dif16 plan with n = 1024; tw_r16 is the plan's
+// lane-major twiddle block.
+hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
+                             uint64_t batch, const dc64 *tw_r16, const dc64 *twist, dc64 *F,
+                             hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  static std::mutex mu;
+  static std::set<int> ready;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    if (!ready.count(dev)) {
+      e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_exact_fwd_r16),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXLds);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_exact_inv_r16),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXLds);
+      if (e != hipSuccess) return e;
+      ready.insert(dev);
+    }
+  }
+  const c64 *t = reinterpret_cast<const c64 *>(tw_r16);
+  const c64 *z = reinterpret_cast<const c64 *>(twist);
+  c64 *f = reinterpret_cast<c64 *>(F);
+  const uint64_t units = batch * 4;
+  hipLaunchKernelGGL(k_exact_fwd_r16, dim3((uint32_t)((units + kXW - 1) / kXW)), dim3(64 * kXW),
+                     kXLds, s, a, b, f, margin, batch, t, z);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t per_wg = kXW / 4;  // products per workgroup (4 waves each)
+  hipLaunchKernelGGL(k_exact_inv_r16, dim3((uint32_t)((batch + per_wg - 1) / per_wg)),
+                     dim3(64 * kXW), kXLds, s, f, c, margin, batch, t, z);
+  return hipGetLastError();
 }
 
 }  // namespace cfft_launch
