@@ -16,8 +16,7 @@
 // an integer and accumulates 2^(16 k) D_k into C (owned by one workgroup).
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-#include <set>
+#include "lds_attr.hpp"
 
 #include "launch.hpp"
 #include "r16_exec.hpp"
@@ -248,20 +247,7 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
 }
 
 hipError_t set_lds(const void *kernel, size_t bytes) {
-  static std::mutex mu;
-  static std::set<const void *> done;
-  if (bytes <= 64 * 1024) return hipSuccess;
-  std::lock_guard<std::mutex> gd(mu);
-  if (done.count(kernel)) return hipSuccess;
-  int dev = 0, maxb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e != hipSuccess) return e;
-  if ((size_t)maxb < bytes) return hipErrorInvalidValue;
-  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, maxb);
-  if (e != hipSuccess) return e;
-  done.insert(kernel);
-  return hipSuccess;
+  return cfft_launch::ensure_lds(kernel, bytes);
 }
 
 struct Geom {
@@ -330,23 +316,10 @@ hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, 
                              uint64_t batch, const dc64 *tw_r16, const dc64 *twist, dc64 *F,
                              hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  static std::mutex mu;
-  static std::set<int> ready;
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
+  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_fwd_r16), kXLds);
+  if (e == hipSuccess)
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_inv_r16), kXLds);
   if (e != hipSuccess) return e;
-  {
-    std::lock_guard<std::mutex> g(mu);
-    if (!ready.count(dev)) {
-      e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_exact_fwd_r16),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXLds);
-      if (e == hipSuccess)
-        e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_exact_inv_r16),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXLds);
-      if (e != hipSuccess) return e;
-      ready.insert(dev);
-    }
-  }
   const c64 *t = reinterpret_cast<const c64 *>(tw_r16);
   const c64 *z = reinterpret_cast<const c64 *>(twist);
   c64 *f = reinterpret_cast<c64 *>(F);

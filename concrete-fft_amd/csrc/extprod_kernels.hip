@@ -11,8 +11,8 @@
 // (level 0 most significant).
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-#include <set>
+#include "lds_attr.hpp"
+
 
 #include "launch.hpp"
 #include "stockham_exec.hpp"
@@ -200,20 +200,7 @@ __global__ void __launch_bounds__(512) k_extprod_cols(const c64 *__restrict__ ke
 }
 
 hipError_t set_lds(const void *kernel, size_t bytes) {
-  static std::mutex mu;
-  static std::set<const void *> done;
-  if (bytes <= 64 * 1024) return hipSuccess;
-  std::lock_guard<std::mutex> gd(mu);
-  if (done.count(kernel)) return hipSuccess;
-  int dev = 0, maxb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e != hipSuccess) return e;
-  if ((size_t)maxb < bytes) return hipErrorInvalidValue;
-  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, maxb);
-  if (e != hipSuccess) return e;
-  done.insert(kernel);
-  return hipSuccess;
+  return cfft_launch::ensure_lds(kernel, bytes);
 }
 
 struct Geom {

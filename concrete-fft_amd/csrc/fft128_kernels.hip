@@ -16,8 +16,8 @@
 // rounds.  The first round reads global memory and the last writes it.
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-#include <set>
+#include "lds_attr.hpp"
+
 
 namespace {
 
@@ -421,20 +421,7 @@ __global__ void __launch_bounds__(256) k_exact128_combine(const double *__restri
 }
 
 hipError_t set_lds(const void *kernel, size_t bytes) {
-  static std::mutex mu;
-  static std::set<const void *> done;
-  if (bytes <= 64 * 1024) return hipSuccess;
-  std::lock_guard<std::mutex> g(mu);
-  if (done.count(kernel)) return hipSuccess;
-  int dev = 0, maxb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e != hipSuccess) return e;
-  if ((size_t)maxb < bytes) return hipErrorInvalidValue;
-  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, maxb);
-  if (e != hipSuccess) return e;
-  done.insert(kernel);
-  return hipSuccess;
+  return cfft_launch::ensure_lds(kernel, bytes);
 }
 
 }  // namespace

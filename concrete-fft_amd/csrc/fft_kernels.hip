@@ -3,8 +3,8 @@
 // negacyclic product.
 #include <hip/hip_runtime.h>
 
-#include <mutex>
-#include <set>
+#include "lds_attr.hpp"
+
 
 #include "launch.hpp"
 #include "stockham_exec.hpp"
@@ -143,20 +143,7 @@ __global__ void __launch_bounds__(512) k_polymul_generic(
 
 // Dynamic LDS above 64 KiB must be opted into per kernel.
 hipError_t set_lds(const void *kernel, size_t bytes) {
-  static std::mutex mu;
-  static std::set<const void *> done;
-  if (bytes <= 64 * 1024) return hipSuccess;
-  std::lock_guard<std::mutex> g(mu);
-  if (done.count(kernel)) return hipSuccess;
-  int dev = 0, maxb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e != hipSuccess) return e;
-  if ((size_t)maxb < bytes) return hipErrorInvalidValue;
-  e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, maxb);
-  if (e != hipSuccess) return e;
-  done.insert(kernel);
-  return hipSuccess;
+  return cfft_launch::ensure_lds(kernel, bytes);
 }
 
 // workgroup geometry of the LDS kernels: E points per thread, about 34 KiB of

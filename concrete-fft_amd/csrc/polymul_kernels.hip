@@ -1,13 +1,16 @@
 // polymul_kernels.hip -- register-resident kernels for Dif16 plans with
-// base_n == n, 512 <= n <= 4096: plain forward / inverse transforms and the
-// fused negacyclic product (N = 2n).
+// base_n == n, 512 <= n <= 4096, and Dif8 plans with n = 1024: plain forward /
+// inverse transforms and the fused negacyclic product (N = 2n).
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+
 #include "launch.hpp"
+#include "lds_attr.hpp"
 #include "r16_exec.hpp"
 #include "r8_exec.hpp"
 
-#include <cstdlib>
 
 using namespace cfft_dev;
 using cfft_launch::dc64;
@@ -173,19 +176,26 @@ hipError_t launch_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t strid
 hipError_t launch_polymul_persistent(const uint64_t *a, const int64_t *b, uint64_t *c,
                                     uint64_t batch, const dc64 *tw, const dc64 *twist,
                                     hipStream_t s) {
-  static int cus = 0;
-  static bool lds_ok = false;
-  if (!lds_ok) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
-      return hipErrorInvalidDevice;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_r16_polymul_persistent),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)kPersistLds);
-    if (e != hipSuccess) return e;
-    lds_ok = true;
+  // per device: compute-unit count and the kernel's LDS limit, set once
+  static std::mutex mu;
+  static std::map<int, int> cus_of;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cus_of.find(dev);
+    if (it == cus_of.end()) {
+      e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (e != hipSuccess) return e;
+      if (cus <= 0) return hipErrorInvalidDevice;
+      e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_r16_polymul_persistent),
+                                  kPersistLds);
+      if (e != hipSuccess) return e;
+      cus_of[dev] = cus;
+    } else {
+      cus = it->second;
+    }
   }
   const uint64_t want = (batch + kPW - 1) / kPW;
   const uint32_t grid = (uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus);

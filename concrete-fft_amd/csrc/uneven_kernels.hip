@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
+#include "lds_attr.hpp"
 #include "stockham_exec.hpp"
 #include "uneven.hpp"
 
@@ -124,11 +125,10 @@ __global__ void k_reset_counter(uint32_t *counter) {
 namespace cfft_ub {
 
 hipError_t prepare() {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_uneven_persistent),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_persistent),
+                                        kLdsBytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(k_uneven_tiles),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes);
+  return cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_tiles), kLdsBytes);
 }
 
 hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
