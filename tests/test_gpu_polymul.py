@@ -99,3 +99,16 @@ def test_polymul_small_outputs_bit_exact(cfft, oracle):
     ref = oracle.polymul_f64(a, b, 6, 1024)
     assert np.array_equal(c, ref), int(np.sum(c != ref))
     assert not c[0].any()
+
+
+@pytest.mark.parametrize("batch", [1, 7, 8, 9, 2049])
+def test_persistent_grid_boundaries(cfft, oracle, batch):
+    # N = 2048 Dif16 / 1024 runs the persistent kernel: 8 products per
+    # workgroup, at most one workgroup per CU, products strided over waves
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    a, b = rand_inputs(batch, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 1024))

@@ -128,3 +128,42 @@ def test_poisson_replay(cfft):
         assert np.array_equal(made[i][1][-1], alone), i
     print("p50 %.1f us p99 %.1f us" % (np.percentile(lat, 50) * 1e6, np.percentile(lat, 99) * 1e6))
     srv.close()
+
+
+def test_concurrent_submitters(cfft):
+    # four host threads submit and wait at once; every result equals the
+    # same request served alone
+    import threading
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(8)
+    srv = Server()
+    keys = _setup(srv, rng)
+    reqs = _requests(srv, keys, rng, 160)
+    alone = []
+    for r in reqs:
+        q, keep = _make(srv, keys, r)
+        alone.append(srv.wait(srv.submit(q, keep)).copy())
+    out = [None] * len(reqs)
+    errors = []
+    lock = threading.Lock()
+
+    def worker(idx):
+        try:
+            for i in idx:
+                q, keep = _make(srv, keys, reqs[i])
+                with lock:  # the wrapper's ticket table is a plain dict
+                    t = srv.submit(q, keep)
+                res = srv.wait(t)
+                out[i] = res
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(range(k, len(reqs), 4),)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i in range(len(reqs)):
+        assert np.array_equal(out[i], alone[i]), i
+    srv.close()

@@ -114,3 +114,18 @@ def test_uneven_full_workload_roundtrip(cfft):
         e_x = np.sum(x[o:o + N] ** 2)
         e_f = np.sum(np.abs(F[f:f + N // 2]) ** 2) / (N // 2)
         assert abs(e_f - e_x) < 1e-12 * e_x
+
+
+@pytest.mark.parametrize("log2_N", [[14], [11] * 300, [5, 14], [9]])
+def test_uneven_degenerate_schedules(cfft, oracle, log2_N):
+    from cfft_amd.device import DeviceBuffer
+    from cfft_amd.uneven import Schedule, packed_jobs
+    rng = np.random.default_rng(len(log2_N))
+    jobs, nin, nf, nout = packed_jobs(log2_N)
+    x = rng.uniform(-1, 1, nin)
+    sched = Schedule(jobs, nin, nf, nout, measure=True)
+    dx = DeviceBuffer.from_host(x)
+    dF = DeviceBuffer((nf,), np.complex128)
+    dy = DeviceBuffer((nout,), np.float64)
+    sched.run(dx, dF, dy)
+    _check_against_oracle(oracle, sched, jobs, x, dF.download(), dy.download())
