@@ -28,7 +28,9 @@ arrival to result, achieved requests/s, batching.  `exact` times the exact
 u64 product (both operands uniform over [0, 2^64), N = 2048 and 4096, batch
 2048) through f64 transforms of 16-bit limbs against the double-double
 transforms of 32-bit limbs, with both results checked equal and a sample
-checked against the u64 schoolbook.  `--mode copy` runs one
+checked against the u64 schoolbook.  `ordered` times the ordered c64 plan at
+n = 1024, batch 1 (fwd and inv per call) and reports its error against the
+oracle and its round trip.  `--mode copy` runs one
 device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
 byte counters).
 """
@@ -265,6 +267,37 @@ def bench_exact(steps, warmup):
     return out
 
 
+def bench_ordered(steps=200):
+    """The ordered c64 plan at n = 1024, batch 1 (latency-bound): fwd then inv
+    then 1/n on re, im uniform in [-1, 1) (seed 1); per-call device time, rel
+    L2 error of fwd against the oracle's restatement of the same plan, round
+    trip error."""
+    from oracle_lib import load_oracle
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.ordered import FftAlgo, Method, Plan
+    n = 1024
+    rng = np.random.default_rng(1)
+    x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(np.complex128)
+    s = Stream()
+    out = {}
+    for algo in (FftAlgo.Dif16, FftAlgo.Dif4):
+        plan = Plan(n, Method.UserProvided(algo))
+        d = DeviceBuffer.from_host(x, s)
+        ms_f = float(np.median(time_runs(lambda: plan.fwd(d, s), s, steps, 10)))
+        ms_i = float(np.median(time_runs(lambda: plan.inv(d, s), s, steps, 10)))
+        y = x.copy()
+        plan.fwd(y)
+        ref = load_oracle().ordered_fwd(x, int(algo))
+        rel = float(np.linalg.norm(y - ref) / np.linalg.norm(ref))
+        plan.inv(y)
+        rt = float(np.abs(y / n - x).max())
+        out[algo.name] = {"fwd_us": round(ms_f * 1e3, 2), "inv_us": round(ms_i * 1e3, 2),
+                          "rel_l2_vs_oracle": rel, "bit_exact_vs_oracle": rel == 0.0,
+                          "roundtrip_max_err": rt}
+    return {"workload": "ordered c64 plan, n = 1024, batch 1, fwd / inv per call (device "
+                        "buffer, HIP events)", "plans": out}
+
+
 def copy_probe(mib):
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -289,8 +322,8 @@ def main():
                     help="products timed in the one-call-per-polynomial form")
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
-    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "copy"],
-                    default="all")
+    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered",
+                                       "copy"], default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=128)
@@ -321,6 +354,9 @@ def main():
         return 0
     if args.mode == "exact":
         print(json.dumps({"mode": "exact", "exact": bench_exact(10, 2)}))
+        return 0
+    if args.mode == "ordered":
+        print(json.dumps({"mode": "ordered", "ordered": bench_ordered()}))
         return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
@@ -387,6 +423,7 @@ def main():
         out["uneven"] = bench_uneven(args.uneven_steps, 2)
         out["serve"] = bench_serve(args.serve_seconds)
         out["exact"] = bench_exact(10, 2)
+        out["ordered"] = bench_ordered()
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
