@@ -30,7 +30,9 @@ u64 product (both operands uniform over [0, 2^64), N = 2048 and 4096, batch
 transforms of 32-bit limbs, with both results checked equal and a sample
 checked against the u64 schoolbook.  `ordered` times the ordered c64 plan at
 n = 1024, batch 1 (fwd and inv per call) and reports its error against the
-oracle and its round trip.  `--mode copy` runs one
+oracle and its round trip.  `transforms` reports batched unordered forward
+transforms (n = 1024, batch 8192) per second for several plans.  `--mode copy`
+runs one
 device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
 byte counters).
 """
@@ -267,6 +269,29 @@ def bench_exact(steps, warmup):
     return out
 
 
+def bench_transforms(steps=50):
+    """Batched unordered forward transforms, n = 1024 (N = 2048), batch 8192,
+    in place on device: transforms/s and HBM fraction at 32 n bytes each."""
+    from cfft_amd import check, lib
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    n, batch = 1024, 8192
+    rng = np.random.default_rng(6)
+    s = Stream()
+    x = DeviceBuffer.from_host(rng.uniform(-1, 1, (batch, 2 * n)).view(np.complex128), s)
+    out = {}
+    for algo, base in ((FftAlgo.Dif16, 1024), (FftAlgo.Dif8, 1024), (FftAlgo.Dit16, 1024),
+                       (FftAlgo.Dif16, 512)):
+        plan = Plan(n, Method.UserProvided(algo, base))
+        f = lambda: check(lib.cfft_fwd(plan.handle, x.dptr, batch, n, s.handle))
+        ms = float(np.median(time_runs(f, s, steps, 3)))
+        gbs = 32 * n * batch / (ms * 1e-3) / 1e9
+        out["%s/%d" % (algo.name, base)] = {"transforms_per_s": round(batch / (ms * 1e-3), 1),
+                                           "ms": round(ms, 4),
+                                           "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return {"workload": "unordered fwd, n = 1024, batch 8192, in place on device", "plans": out}
+
+
 def bench_ordered(steps=200):
     """The ordered c64 plan at n = 1024, batch 1 (latency-bound): fwd then inv
     then 1/n on re, im uniform in [-1, 1) (seed 1); per-call device time, rel
@@ -323,7 +348,7 @@ def main():
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
     ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered",
-                                       "copy"], default="all")
+                                       "transforms", "copy"], default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--copy-mib", type=int, default=128)
@@ -357,6 +382,9 @@ def main():
         return 0
     if args.mode == "ordered":
         print(json.dumps({"mode": "ordered", "ordered": bench_ordered()}))
+        return 0
+    if args.mode == "transforms":
+        print(json.dumps({"mode": "transforms", "transforms": bench_transforms()}))
         return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
@@ -424,6 +452,7 @@ def main():
         out["serve"] = bench_serve(args.serve_seconds)
         out["exact"] = bench_exact(10, 2)
         out["ordered"] = bench_ordered()
+        out["transforms"] = bench_transforms()
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
