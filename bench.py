@@ -131,11 +131,13 @@ def bench_uneven(steps, warmup):
     tuned = Schedule(jobs, nin, nf, nout, measure=True)
     tune_s = time.perf_counter() - t0
     fixed = Schedule(jobs, nin, nf, nout, measure=False, algo=FftAlgo.Dif4)
+    dif16 = Schedule(jobs, nin, nf, nout, measure=False, algo=FftAlgo.Dif16)
     forms = {
         "tuned_persistent_graph": lambda: tuned.run(dx, dF, dy, s),
         "tuned_plain": lambda: tuned.run_plain(dx, dF, dy, s),
         "fixed_persistent_graph": lambda: fixed.run(dx, dF, dy, s),
         "fixed_plain": lambda: fixed.run_plain(dx, dF, dy, s),
+        "dif16_persistent_graph": lambda: dif16.run(dx, dF, dy, s),
     }
     ms = {k: float(np.median(time_runs(f, s, steps, warmup))) for k, f in forms.items()}
     tuned.run(dx, dF, dy, s)

@@ -14,6 +14,10 @@ constexpr uint32_t kThreads = 512;  // one workgroup: 8 waves
 // LDS of one workgroup: (8192 / n) transform groups of n + n/16 points
 constexpr uint32_t kLdsPoints = 8192 + 512;
 constexpr size_t kLdsBytes = kLdsPoints * 16;
+// Dif16 body: tile data followed by the class's base twiddle block
+// (rows k < 16 of T[k l], l < 64, then rows of T[16 k p], p < 4)
+constexpr uint32_t kTwR16Points = 16 * 64 + 16 * 4;
+constexpr size_t kR16LdsBytes = (size_t)(kLdsPoints + kTwR16Points) * 16;
 constexpr uint32_t kMinLog2N = 5;   // N = 32: n = 16, one thread per transform
 constexpr uint32_t kMaxLog2N = 14;  // N = 16384: n = 8192, one transform per workgroup
 
@@ -38,20 +42,32 @@ struct UClass {
   const cfft_launch::dc64 *tw;  // T[j], n entries
   const cfft_launch::dc64 *twist;
   const cfft_launch::dc64 *untwist;
+  // Dif16 plans with base_n = min(n, 1024), 512 <= n <= 8192: twiddle block
+  // of the base transform (T[k l tw_mul] as [k][l], then T[16 k p tw_mul] as
+  // [k][p], tw_mul = n / base_n), null for classes run by the generic body
+  const cfft_launch::dc64 *tw_r16;
   uint32_t log2n;
-  uint32_t pad;
+  uint32_t r16;  // 1: the class runs the Dif16 body
 };
+
+// classes the Dif16 body runs: Dif16 plan, base_n = min(n, 1024), 9 <= log2n <= 13
+constexpr bool r16_class(bool dif16, uint32_t log2n, uint32_t log2_base) {
+  return dif16 && log2n >= 9 && log2n <= 13 && log2_base == (log2n < 10 ? log2n : 10);
+}
 
 // groups (jobs) per tile for a class of complex length 2^log2n
 constexpr uint32_t groups_per_tile(uint32_t log2n) { return kThreads / ((1u << log2n) / 16); }
 
 // sets the kernels' LDS limit on the current device (before any launch)
 hipError_t prepare();
-hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+// Resets `counter`, then runs tiles[0 .. ntiles) on `grid` persistent
+// workgroups; `r16` selects the Dif16 body (every tile's class must have r16
+// set) or the generic body.
+hipError_t launch_persistent(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
                              uint32_t ntiles, const UClass *classes, uint32_t *counter,
                              uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
                              double *out, hipStream_t s);
-hipError_t launch_tiles(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+hipError_t launch_tiles(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
                         uint32_t first_tile, uint32_t count, const UClass *classes,
                         const double *in, cfft_launch::dc64 *fourier, double *out, hipStream_t s);
 

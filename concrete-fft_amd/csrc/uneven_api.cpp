@@ -8,6 +8,9 @@
 // the fastest of a candidate set, timed on the device with this schedule's
 // own kernels), tiles of up to 8192/n jobs of one class, queued largest class
 // first so the long tiles start early and the short ones fill the tail.
+// Classes whose plan is Dif16 with base_n = min(n, 1024) (512 <= n <= 8192)
+// run the register-resident Dif16 body in their own persistent launch (their
+// tiles come first in the queue); the others run the generic body.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,6 +19,7 @@
 #include <vector>
 
 #include "cfft.h"
+#include "host_tables.hpp"
 #include "plan.hpp"
 #include "uneven.hpp"
 
@@ -33,13 +37,16 @@ struct DevSchedule {
   uint32_t *order = nullptr;
   UTile *tiles = nullptr;
   UClass *classes = nullptr;
-  uint32_t *counter = nullptr;
+  dc64 *tw_r16 = nullptr;      // base twiddle blocks of the Dif16 classes
+  uint32_t *counter = nullptr;  // [0]: Dif16 queue, [32]: generic queue
   uint32_t ntiles = 0;
+  uint32_t ntiles_r16 = 0;      // tiles [0, ntiles_r16) run the Dif16 body
   void release() {
     if (jobs) (void)hipFree(jobs);
     if (order) (void)hipFree(order);
     if (tiles) (void)hipFree(tiles);
     if (classes) (void)hipFree(classes);
+    if (tw_r16) (void)hipFree(tw_r16);
     if (counter) (void)hipFree(counter);
     *this = DevSchedule{};
   }
@@ -47,14 +54,16 @@ struct DevSchedule {
 
 struct ClassRange {
   uint32_t log2n = 0;
+  bool r16 = false;
   uint32_t first_tile = 0, ntiles = 0;
 };
 
-// tiles of `order` (job indices grouped by class, classes in `class_seq`)
+// tiles of `order` (job indices grouped by class)
 struct HostSchedule {
   std::vector<uint32_t> order;
   std::vector<UTile> tiles;
   std::vector<ClassRange> ranges;  // per class, in queue order
+  uint32_t ntiles_r16 = 0;
 };
 
 template <class T>
@@ -65,22 +74,25 @@ hipError_t upload(T **dst, const std::vector<T> &src) {
   return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
-// jobs_by_class[c]: job indices of class c (complex length 2^log2n[c]);
-// classes are queued in descending size
+// jobs_by_class[c]: job indices of class c; classes of the Dif16 body are
+// queued first, each group in descending size
 HostSchedule make_schedule(const std::vector<std::vector<uint32_t>> &jobs_by_class,
-                           const std::vector<uint32_t> &class_log2n) {
+                           const std::vector<UClass> &classes) {
   HostSchedule h;
   std::vector<uint32_t> seq(jobs_by_class.size());
   for (uint32_t c = 0; c < seq.size(); ++c) seq[c] = c;
-  std::stable_sort(seq.begin(), seq.end(),
-                   [&](uint32_t a, uint32_t b) { return class_log2n[a] > class_log2n[b]; });
+  std::stable_sort(seq.begin(), seq.end(), [&](uint32_t a, uint32_t b) {
+    if (classes[a].r16 != classes[b].r16) return classes[a].r16 > classes[b].r16;
+    return classes[a].log2n > classes[b].log2n;
+  });
   for (uint32_t c : seq) {
     const auto &js = jobs_by_class[c];
     if (js.empty()) continue;
     ClassRange r;
-    r.log2n = class_log2n[c];
+    r.log2n = classes[c].log2n;
+    r.r16 = classes[c].r16 != 0;
     r.first_tile = (uint32_t)h.tiles.size();
-    const uint32_t G = cfft_ub::groups_per_tile(class_log2n[c]);
+    const uint32_t G = cfft_ub::groups_per_tile(classes[c].log2n);
     const uint32_t base = (uint32_t)h.order.size();
     h.order.insert(h.order.end(), js.begin(), js.end());
     for (uint32_t i = 0; i < js.size(); i += G) {
@@ -92,19 +104,51 @@ HostSchedule make_schedule(const std::vector<std::vector<uint32_t>> &jobs_by_cla
       h.tiles.push_back(t);
     }
     r.ntiles = (uint32_t)h.tiles.size() - r.first_tile;
+    if (r.r16) h.ntiles_r16 += r.ntiles;
     h.ranges.push_back(r);
   }
   return h;
 }
 
+// base twiddle block of a Dif16 class (layout of UClass::tw_r16)
+std::vector<cfft_host::C64> r16_block(const cfft_plan_s *p) {
+  const size_t n = p->n, base = p->base_n, mul = n / base;
+  const size_t T = base / 16, T2 = T / 16;
+  const std::vector<cfft_host::C64> tw = cfft_host::twiddle_table(n);
+  std::vector<cfft_host::C64> blk(16 * T + 16 * T2);
+  for (size_t k = 0; k < 16; ++k) {
+    for (size_t l = 0; l < T; ++l) blk[k * T + l] = tw[k * l * mul];
+    for (size_t q = 0; q < T2; ++q) blk[16 * T + k * T2 + q] = tw[16 * k * q * mul];
+  }
+  return blk;
+}
+
+// `plans[c]` is the plan of classes[c]; fills in the Dif16 twiddle blocks
 hipError_t upload_schedule(DevSchedule &d, const std::vector<UJob> &jobs, const HostSchedule &h,
-                           const std::vector<UClass> &classes) {
+                           std::vector<UClass> classes,
+                           const std::vector<const cfft_plan_s *> &plans) {
+  std::vector<cfft_host::C64> blocks;
+  std::vector<size_t> at(classes.size(), 0);
+  for (size_t c = 0; c < classes.size(); ++c) {
+    if (!classes[c].r16) continue;
+    at[c] = blocks.size();
+    const std::vector<cfft_host::C64> b = r16_block(plans[c]);
+    blocks.insert(blocks.end(), b.begin(), b.end());
+  }
   hipError_t e = upload(&d.jobs, jobs);
   if (e == hipSuccess) e = upload(&d.order, h.order);
   if (e == hipSuccess) e = upload(&d.tiles, h.tiles);
+  if (e == hipSuccess && !blocks.empty()) {
+    e = hipMalloc(reinterpret_cast<void **>(&d.tw_r16), blocks.size() * sizeof(dc64));
+    if (e == hipSuccess)
+      e = hipMemcpy(d.tw_r16, blocks.data(), blocks.size() * sizeof(dc64), hipMemcpyHostToDevice);
+    for (size_t c = 0; c < classes.size(); ++c)
+      if (classes[c].r16) classes[c].tw_r16 = d.tw_r16 + at[c];
+  }
   if (e == hipSuccess) e = upload(&d.classes, classes);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.counter), 256);
   d.ntiles = (uint32_t)h.tiles.size();
+  d.ntiles_r16 = h.ntiles_r16;
   return e;
 }
 
@@ -115,10 +159,27 @@ UClass class_of(const cfft_plan_s *p) {
   c.tw = p->d_tw;
   c.twist = p->d_twist;
   c.untwist = p->d_untwist;
-  uint32_t l = 0;
+  uint32_t l = 0, lb = 0;
   while (((size_t)1 << l) < p->n) ++l;
+  while (((size_t)1 << lb) < p->base_n) ++lb;
   c.log2n = l;
+  c.tw_r16 = nullptr;
+  c.r16 = cfft_ub::r16_class(p->algo == CFFT_DIF16, l, lb) ? 1u : 0u;
   return c;
+}
+
+// the Dif16 tiles, then the generic tiles, each on its own queue counter
+hipError_t launch_schedule(const DevSchedule &d, uint32_t grid, const double *in, dc64 *f,
+                           double *out, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if (d.ntiles_r16)
+    e = cfft_ub::launch_persistent(true, d.jobs, d.order, d.tiles, d.ntiles_r16, d.classes,
+                                   d.counter, grid, in, f, out, s);
+  if (e == hipSuccess && d.ntiles > d.ntiles_r16)
+    e = cfft_ub::launch_persistent(false, d.jobs, d.order, d.tiles + d.ntiles_r16,
+                                   d.ntiles - d.ntiles_r16, d.classes, d.counter + 32, grid, in,
+                                   f, out, s);
+  return e;
 }
 
 struct Candidate {
@@ -145,7 +206,7 @@ int grid_size(uint32_t *grid) {
   int maxlds = 0;
   e = hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
   if (e != hipSuccess) return hip_error(e, "device query");
-  if ((size_t)maxlds < cfft_ub::kLdsBytes + 64)
+  if ((size_t)maxlds < cfft_ub::kR16LdsBytes + 64)
     return set_error(CFFT_ERROR_SIZE, "device LDS per workgroup is below the uneven kernel's need");
   *grid = (uint32_t)cus;  // the workgroup's LDS fills a CU
   return CFFT_SUCCESS;
@@ -190,13 +251,13 @@ int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t gr
   for (size_t i = 0; i < njobs; ++i) jobs[i] = UJob{2 * n * i, n * i, 2 * n * i, (uint32_t)log2n, 0};
   std::vector<std::vector<uint32_t>> by_class(1);
   for (uint32_t i = 0; i < njobs; ++i) by_class[0].push_back(i);
-  HostSchedule h = make_schedule(by_class, {log2n});
   std::vector<UClass> cls{class_of(p)};
+  HostSchedule h = make_schedule(by_class, cls);
   DevSchedule d;
   double *x = nullptr, *y = nullptr;
   dc64 *f = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e = upload_schedule(d, jobs, h, cls);
+  hipError_t e = upload_schedule(d, jobs, h, cls, {p});
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x), 2 * n * njobs * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&y), 2 * n * njobs * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&f), n * njobs * sizeof(dc64));
@@ -206,9 +267,7 @@ int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t gr
   double best = 1e30;
   for (int it = 0; it < 4 && e == hipSuccess; ++it) {
     e = hipEventRecord(e0, nullptr);
-    if (e == hipSuccess)
-      e = cfft_ub::launch_persistent(d.jobs, d.order, d.tiles, d.ntiles, d.classes, d.counter,
-                                         grid, x, f, y, nullptr);
+    if (e == hipSuccess) e = launch_schedule(d, grid, x, f, y, nullptr);
     if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.f;
@@ -259,6 +318,7 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
   if (ep != hipSuccess) return hip_error(ep, "uneven kernel attributes");
   std::vector<std::vector<uint32_t>> by_class;
   std::vector<UClass> classes;
+  std::vector<const cfft_plan_s *> class_plans;
   for (uint32_t l = 0; l < by_log2n.size(); ++l) {
     if (by_log2n[l].empty()) continue;
     const size_t n = (size_t)1 << l;
@@ -284,10 +344,11 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
     u->plans.push_back(p);
     u->class_log2n.push_back(l);
     classes.push_back(class_of(p));
+    class_plans.push_back(p);
     by_class.push_back(by_log2n[l]);
   }
-  u->host = make_schedule(by_class, u->class_log2n);
-  hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes);
+  u->host = make_schedule(by_class, classes);
+  hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes, class_plans);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   return hip_error(e, "uneven schedule upload");
@@ -366,9 +427,8 @@ int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *ou
     u->graph = nullptr;
     e = hipStreamBeginCapture(u->cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipError_t el = cfft_ub::launch_persistent(
-          u->dev.jobs, u->dev.order, u->dev.tiles, u->dev.ntiles, u->dev.classes, u->dev.counter,
-          u->grid, in, reinterpret_cast<dc64 *>(fourier), out, u->cap);
+      hipError_t el =
+          launch_schedule(u->dev, u->grid, in, reinterpret_cast<dc64 *>(fourier), out, u->cap);
       e = hipStreamEndCapture(u->cap, &u->graph);
       if (el != hipSuccess) e = el;
     }
@@ -387,10 +447,10 @@ int cfft_uneven_run_plain(cfft_uneven u, const double *in, double *fourier, doub
   if (u->jobs.empty()) return CFFT_SUCCESS;
   if (!in || !fourier || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   for (const ClassRange &r : u->host.ranges) {
-    hipError_t e = cfft_ub::launch_tiles(u->dev.jobs, u->dev.order, u->dev.tiles,
-                                             r.first_tile, r.ntiles, u->dev.classes, in,
-                                             reinterpret_cast<dc64 *>(fourier), out,
-                                             (hipStream_t)stream);
+    hipError_t e = cfft_ub::launch_tiles(r.r16, u->dev.jobs, u->dev.order, u->dev.tiles,
+                                         r.first_tile, r.ntiles, u->dev.classes, in,
+                                         reinterpret_cast<dc64 *>(fourier), out,
+                                         (hipStream_t)stream);
     if (e != hipSuccess) return hip_error(e, "uneven plain launch");
   }
   return CFFT_SUCCESS;

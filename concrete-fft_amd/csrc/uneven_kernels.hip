@@ -12,14 +12,22 @@
 // across CUs (tiles are queued largest first).  The plain form runs the same
 // tile body with one launch per class and one workgroup per tile.
 //
-// The tile body is the pass-program executor (any plan, 16 points per
-// thread).  The register-resident Dif16 schedule is not used here: inlined
-// beside the generic body in the persistent loop it pushes the kernel past
-// 256 VGPRs (spills), and as a called function it needs a scratch stack.
+// Two tile bodies, in two kernels (inlined side by side in one persistent
+// loop they need more than 256 VGPRs):
+//  - generic: the pass-program executor (any plan, 16 points per thread),
+//    a workgroup barrier around every pass;
+//  - Dif16 (plans Dif16 with base_n = min(n, 1024), 512 <= n <= 8192): the
+//    register-resident Dif16 schedule on one wavefront per 1024-point base
+//    transform.  For n > 1024 the plan's single depth pass (radix n / 1024,
+//    src/unordered.rs:391-489) runs in registers right after the loads
+//    (forward) and right before the stores (inverse), so a tile needs two
+//    workgroup barriers; for n <= 1024 a tile is wave-local.  The base
+//    transform's twiddle block is staged in LDS once per class change.
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
 #include "lds_attr.hpp"
+#include "r16_exec.hpp"
 #include "stockham_exec.hpp"
 #include "uneven.hpp"
 
@@ -90,6 +98,211 @@ __device__ __forceinline__ void run_tile(const UClass &C, const UTile &tile,
   tile_generic(C, tile, jobs, order, in, fourier, out, smem);
 }
 
+// ---- Dif16 body ----
+
+// n = 2^LOG2N <= 1024: one job on T = n/16 lanes of one wavefront
+template <int LOG2N>
+__device__ __forceinline__ void tile_r16_local(const UClass &C, const UTile &tile,
+                                               const UJob *__restrict__ jobs,
+                                               const uint32_t *__restrict__ order,
+                                               const double *in, c64 *fourier, double *out,
+                                               c64 *smem, const c64 *twl, uint32_t tid) {
+  using S = R16<LOG2N>;
+  constexpr uint32_t n = S::n, T = S::T;
+  const uint32_t g = tid / T, l = tid % T;
+  const bool active = g < tile.count;
+  c64 *x = smem + g * S::lds_words;
+  JobPtrs P{nullptr, nullptr, nullptr};
+  if (active) P = job_ptrs(jobs[order[tile.first + g]], in, fourier, out);
+  const c64 *twist = reinterpret_cast<const c64 *>(C.twist);
+  const c64 *untwist = reinterpret_cast<const c64 *>(C.untwist);
+  c64 v[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    const uint32_t j = l + T * m;
+    v[m] = active ? cmul(twist[j], mk(ld_stream(&P.x[j]), ld_stream(&P.x[j + n]))) : mk(0.0, 0.0);
+  }
+  r16l_transform<LOG2N, true>(v, x, l, twl);
+  if (active) {
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &P.f[l + T * S::end_m(s)]);
+  }
+  c64 u[16];
+  r16_end_to_input<LOG2N>(v, u);
+  r16l_transform<LOG2N, false>(u, x, l, twl);
+  if (active) {
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + T * S::end_m(s);
+      const c64 r = cmul(untwist[j], u[s]);
+      st_stream(r.re, &P.y[j]);
+      st_stream(r.im, &P.y[j + n]);
+    }
+  }
+}
+
+// n = 2^LOG2N = R * 1024 (R = 2, 4, 8): one job on R wavefronts.  Depth
+// pass (DEPTH_FWD, radix R over the whole job; butterfly j < 1024 reads the
+// points k * 1024 + j, its output k, times T[k j], goes to chunk bitrev(k)),
+// then wave c runs the base transform of chunk c, forward and inverse, then
+// the inverse depth pass (DEPTH_INV: conj T[k j] before the butterfly).
+// Chunk c lives in LDS at c * 1088 points (r16 padding); the base transform
+// uses that same region as its exchange buffer.
+template <int LOG2N>
+__device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &tile,
+                                               const UJob *__restrict__ jobs,
+                                               const uint32_t *__restrict__ order,
+                                               const double *in, c64 *fourier, double *out,
+                                               c64 *smem, const c64 *twl, uint32_t tid) {
+  using S = R16<10>;
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr int R = 1 << (LOG2N - 10);
+  constexpr uint32_t TJ = n / 16;        // threads per job
+  constexpr uint32_t NBD = 16 / R;       // depth butterflies per thread
+  constexpr uint32_t CH = S::lds_words;  // LDS points per chunk
+  const uint32_t g = tid / TJ, t = tid % TJ;
+  const bool active = g < tile.count;
+  c64 *X = smem + g * (R * CH);
+  JobPtrs P{nullptr, nullptr, nullptr};
+  if (active) P = job_ptrs(jobs[order[tile.first + g]], in, fourier, out);
+  const c64 *tw = reinterpret_cast<const c64 *>(C.tw);
+  const c64 *twist = reinterpret_cast<const c64 *>(C.twist);
+  const c64 *untwist = reinterpret_cast<const c64 *>(C.untwist);
+  {
+    c64 v[NBD][R];
+#pragma unroll
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t p = (uint32_t)k * 1024 + j;
+        v[i][k] = active ? cmul(twist[p], mk(ld_stream(&P.x[p]), ld_stream(&P.x[p + n])))
+                         : mk(0.0, 0.0);
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+      butterfly<R, true>(v[i]);
+#pragma unroll
+      for (int k = 1; k < R; ++k) v[i][k] = cmul(tw[(uint32_t)k * j], v[i][k]);
+#pragma unroll
+      for (int k = 0; k < R; ++k) X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)] = v[i][k];
+    }
+  }
+  __syncthreads();
+  {
+    const uint32_t c = t >> 6, l = t & 63;
+    c64 *xc = X + c * CH;
+    c64 v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) v[m] = xc[r16_pad(l + 64 * m)];
+    r16l_transform<10, true>(v, xc, l, twl);
+    if (active) {
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &P.f[c * 1024 + l + 64 * S::end_m(s)]);
+    }
+    c64 u[16];
+    r16_end_to_input<10>(v, u);
+    r16l_transform<10, false>(u, xc, l, twl);
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
+  }
+  __syncthreads();
+  {
+    c64 v[NBD][R];
+#pragma unroll
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+#pragma unroll
+      for (int k = 0; k < R; ++k) v[i][k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        c64 w = tw[(uint32_t)k * j];
+        w.im = -w.im;
+        v[i][k] = cmul(w, v[i][k]);
+      }
+      butterfly<R, false>(v[i]);
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const uint32_t p = (uint32_t)k * 1024 + j;
+          const c64 r = cmul(untwist[p], v[i][k]);
+          st_stream(r.re, &P.y[p]);
+          st_stream(r.im, &P.y[p + n]);
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void run_tile_r16(const UClass &C, const UTile &tile,
+                                             const UJob *__restrict__ jobs,
+                                             const uint32_t *__restrict__ order,
+                                             const double *in, c64 *fourier, double *out,
+                                             c64 *smem, const c64 *twl, uint32_t tid) {
+  switch (C.log2n) {
+    case 9: tile_r16_local<9>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+    case 10: tile_r16_local<10>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+    case 11: tile_r16_depth<11>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+    case 12: tile_r16_depth<12>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+    default: tile_r16_depth<13>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+  }
+}
+
+// copies the class's base twiddle block into LDS (callers barrier after)
+__device__ __forceinline__ void stage_r16_twiddles(const UClass &C, c64 *twl) {
+  const uint32_t T = C.log2n >= 10 ? 64u : (1u << C.log2n) / 16;
+  const uint32_t words = 16 * T + T;
+  const c64 *src = reinterpret_cast<const c64 *>(C.tw_r16);
+  for (uint32_t i = threadIdx.x; i < words; i += kThreads) twl[i] = src[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
+    const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
+    const UTile *__restrict__ tiles, uint32_t ntiles, const UClass *__restrict__ classes,
+    uint32_t *counter, const double *in, c64 *fourier, double *out) {
+  extern __shared__ c64 smem[];
+  __shared__ uint32_t s_tile;
+  c64 *twl = smem + cfft_ub::kLdsPoints;
+  uint32_t staged = 0xffffffffu;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t t = s_tile;
+    __syncthreads();
+    if (t >= ntiles) break;
+    const UTile tile = tiles[t];
+    const UClass &C = classes[tile.cls];
+    if (tile.cls != staged) {
+      stage_r16_twiddles(C, twl);
+      __syncthreads();
+      staged = tile.cls;
+    }
+    // opaque per tile: address arithmetic of the five bodies is not hoisted
+    // out of the loop (it would stay live across it and spill)
+    run_tile_r16(C, tile, jobs, order, in, fourier, out, smem, twl, launder_u32(threadIdx.x));
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_uneven_r16_tiles(
+    const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
+    const UTile *__restrict__ tiles, uint32_t first_tile, const UClass *__restrict__ classes,
+    const double *in, c64 *fourier, double *out) {
+  extern __shared__ c64 smem[];
+  c64 *twl = smem + cfft_ub::kLdsPoints;
+  const UTile tile = tiles[first_tile + blockIdx.x];
+  const UClass &C = classes[tile.cls];
+  stage_r16_twiddles(C, twl);
+  __syncthreads();
+  run_tile_r16(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
+}
+
 __global__ void __launch_bounds__(kThreads) k_uneven_persistent(
     const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
     const UTile *__restrict__ tiles, uint32_t ntiles, const UClass *__restrict__ classes,
@@ -127,11 +340,17 @@ namespace cfft_ub {
 hipError_t prepare() {
   hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_persistent),
                                         kLdsBytes);
-  if (e != hipSuccess) return e;
-  return cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_tiles), kLdsBytes);
+  if (e == hipSuccess)
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_tiles), kLdsBytes);
+  if (e == hipSuccess)
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_r16_persistent),
+                                kR16LdsBytes);
+  if (e == hipSuccess)
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_uneven_r16_tiles), kR16LdsBytes);
+  return e;
 }
 
-hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+hipError_t launch_persistent(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
                              uint32_t ntiles, const UClass *classes, uint32_t *counter,
                              uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
                              double *out, hipStream_t s) {
@@ -139,18 +358,28 @@ hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTil
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (ntiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_uneven_persistent, dim3(grid), dim3(kThreads), kLdsBytes, s, jobs, order,
-                     tiles, ntiles, classes, counter, in, reinterpret_cast<c64 *>(fourier), out);
+  c64 *f = reinterpret_cast<c64 *>(fourier);
+  if (r16)
+    hipLaunchKernelGGL(k_uneven_r16_persistent, dim3(grid), dim3(kThreads), kR16LdsBytes, s, jobs,
+                       order, tiles, ntiles, classes, counter, in, f, out);
+  else
+    hipLaunchKernelGGL(k_uneven_persistent, dim3(grid), dim3(kThreads), kLdsBytes, s, jobs, order,
+                       tiles, ntiles, classes, counter, in, f, out);
   return hipGetLastError();
 }
 
-hipError_t launch_tiles(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+hipError_t launch_tiles(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
                         uint32_t first_tile, uint32_t count, const UClass *classes,
                         const double *in, cfft_launch::dc64 *fourier, double *out,
                         hipStream_t s) {
   if (count == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_uneven_tiles, dim3(count), dim3(kThreads), kLdsBytes, s, jobs, order,
-                     tiles, first_tile, classes, in, reinterpret_cast<c64 *>(fourier), out);
+  c64 *f = reinterpret_cast<c64 *>(fourier);
+  if (r16)
+    hipLaunchKernelGGL(k_uneven_r16_tiles, dim3(count), dim3(kThreads), kR16LdsBytes, s, jobs,
+                       order, tiles, first_tile, classes, in, f, out);
+  else
+    hipLaunchKernelGGL(k_uneven_tiles, dim3(count), dim3(kThreads), kLdsBytes, s, jobs, order,
+                       tiles, first_tile, classes, in, f, out);
   return hipGetLastError();
 }
 

@@ -69,6 +69,37 @@ def test_uneven_bit_exact(cfft, oracle, measure):
     assert np.array_equal(dy2.download().view(np.uint64), y.view(np.uint64))
 
 
+def test_uneven_dif16_body_bit_exact(cfft, oracle):
+    # Dif16 plans: N = 1024 .. 16384 run the register-resident Dif16 body
+    # (wave-local for N <= 2048, one depth pass around 1024-point base
+    # transforms above), N <= 512 the generic body, both in one schedule;
+    # partial last tiles in every class
+    from cfft_amd.device import DeviceBuffer, synchronize
+    from cfft_amd.uneven import Schedule
+    from cfft_amd.unordered import FftAlgo
+    rng = np.random.default_rng(29)
+    log2_N = rng.choice(np.arange(5, 15), size=300)
+    jobs, nin, nf, nout = _jobs_with_gaps(log2_N, rng)
+    sched = Schedule(jobs, nin, nf, nout, algo=FftAlgo.Dif16)
+    for l in range(10, 15):
+        assert sched.class_algo(l) == (FftAlgo.Dif16, min(1 << (l - 1), 1024))
+    dx = DeviceBuffer((nin,), np.float64)
+    dF = DeviceBuffer((nf,), np.complex128)
+    dy = DeviceBuffer((nout,), np.float64)
+    for rep in range(2):
+        x = rng.uniform(-1, 1, nin)
+        dx.upload(x)
+        sched.run(dx, dF, dy)
+        synchronize()
+        F, y = dF.download(), dy.download()
+        _check_against_oracle(oracle, sched, jobs, x, F, y)
+    dF2 = DeviceBuffer((nf,), np.complex128)
+    dy2 = DeviceBuffer((nout,), np.float64)
+    sched.run_plain(dx, dF2, dy2)
+    assert np.array_equal(dF2.download().view(np.uint64), F.view(np.uint64))
+    assert np.array_equal(dy2.download().view(np.uint64), y.view(np.uint64))
+
+
 def test_uneven_graph_replays(cfft, oracle):
     from cfft_amd.device import DeviceBuffer, synchronize
     from cfft_amd.uneven import Schedule
