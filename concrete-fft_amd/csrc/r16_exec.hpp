@@ -257,18 +257,31 @@ __device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_
   }
 }
 
-// One transform of n = 2^LOG2N points (LOG2N = 9 or 10) on T = n/16 lanes of
-// one wavefront (two transforms per wave at n = 512, each on its own half).
+// One transform of n = 2^LOG2N points (LOG2N = 8, 9 or 10) on T = n/16
+// lanes of one wavefront (64 / T transforms per wave, each on its own lanes).
 // `tw` is the r16 twiddle block of R16<LOG2N> (in LDS); `x` is the
 // transform's own exchange buffer (R16<LOG2N>::lds_words points).  Same
-// layouts and operation order as r16_transform; no s_barrier.
+// layouts and operation order as r16_transform; no s_barrier.  At n = 256
+// the reference's Dif16 is one core at stride 1 and a radix-16 end at stride
+// 16 (points l + 16 k, in place), so there is one exchange and end_m(s) = s.
 template <int LOG2N, bool FWD>
 __device__ __forceinline__ void r16l_transform(c64 *v, c64 *x, uint32_t l, const c64 *tw) {
   using S = R16<LOG2N>;
-  static_assert(S::T <= 64, "one transform must fit a wavefront");
+  static_assert(S::T <= 64 && LOG2N >= 8, "one transform must fit a wavefront");
   butterfly<16, FWD>(v);
   sched_fence();
   twiddle16_x1<FWD>(v, tw, S::T, l);
+  if constexpr (LOG2N == 8) {
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[r16_pad(16 * l + (uint32_t)k)] = v[k];
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + 16 * (uint32_t)k)];
+    wave_lds_fence();
+    butterfly<16, FWD>(v);
+    return;
+  }
   const uint32_t p = l >> 4, q = l & 15;
   wave_lds_fence();
 #pragma unroll
@@ -290,7 +303,9 @@ __device__ __forceinline__ void r16l_transform(c64 *v, c64 *x, uint32_t l, const
       v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
   wave_lds_fence();
 #pragma unroll
-  for (uint32_t i = 0; i < S::NB; ++i) butterfly<S::REND, FWD>(v + i * S::REND);
+  for (uint32_t i = 0; i < S::NB; ++i) {
+    if constexpr (S::REND > 1) butterfly<S::REND, FWD>(v + i * S::REND);
+  }
 }
 
 }  // namespace cfft_dev

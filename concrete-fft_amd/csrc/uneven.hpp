@@ -42,7 +42,7 @@ struct UClass {
   const cfft_launch::dc64 *tw;  // T[j], n entries
   const cfft_launch::dc64 *twist;
   const cfft_launch::dc64 *untwist;
-  // Dif16 plans with base_n = min(n, 1024), 512 <= n <= 8192: twiddle block
+  // Dif16 plans with base_n = min(n, 1024), 256 <= n <= 8192: twiddle block
   // of the base transform (T[k l tw_mul] as [k][l], then T[16 k p tw_mul] as
   // [k][p], tw_mul = n / base_n), null for classes run by the generic body
   const cfft_launch::dc64 *tw_r16;
@@ -50,9 +50,9 @@ struct UClass {
   uint32_t r16;  // 1: the class runs the Dif16 body
 };
 
-// classes the Dif16 body runs: Dif16 plan, base_n = min(n, 1024), 9 <= log2n <= 13
+// classes the Dif16 body runs: Dif16 plan, base_n = min(n, 1024), 8 <= log2n <= 13
 constexpr bool r16_class(bool dif16, uint32_t log2n, uint32_t log2_base) {
-  return dif16 && log2n >= 9 && log2n <= 13 && log2_base == (log2n < 10 ? log2n : 10);
+  return dif16 && log2n >= 8 && log2n <= 13 && log2_base == (log2n < 10 ? log2n : 10);
 }
 
 // groups (jobs) per tile for a class of complex length 2^log2n

@@ -16,7 +16,7 @@
 // loop they need more than 256 VGPRs):
 //  - generic: the pass-program executor (any plan, 16 points per thread),
 //    a workgroup barrier around every pass;
-//  - Dif16 (plans Dif16 with base_n = min(n, 1024), 512 <= n <= 8192): the
+//  - Dif16 (plans Dif16 with base_n = min(n, 1024), 256 <= n <= 8192): the
 //    register-resident Dif16 schedule on one wavefront per 1024-point base
 //    transform.  For n > 1024 the plan's single depth pass (radix n / 1024,
 //    src/unordered.rs:391-489) runs in registers right after the loads
@@ -100,7 +100,7 @@ __device__ __forceinline__ void run_tile(const UClass &C, const UTile &tile,
 
 // ---- Dif16 body ----
 
-// n = 2^LOG2N <= 1024: one job on T = n/16 lanes of one wavefront
+// n = 2^LOG2N, 256 <= n <= 1024: one job on T = n/16 lanes of one wavefront
 template <int LOG2N>
 __device__ __forceinline__ void tile_r16_local(const UClass &C, const UTile &tile,
                                                const UJob *__restrict__ jobs,
@@ -247,6 +247,7 @@ __device__ __forceinline__ void run_tile_r16(const UClass &C, const UTile &tile,
                                              const double *in, c64 *fourier, double *out,
                                              c64 *smem, const c64 *twl, uint32_t tid) {
   switch (C.log2n) {
+    case 8: tile_r16_local<8>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 9: tile_r16_local<9>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 10: tile_r16_local<10>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 11: tile_r16_depth<11>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
@@ -268,15 +269,19 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
     const UTile *__restrict__ tiles, uint32_t ntiles, const UClass *__restrict__ classes,
     uint32_t *counter, const double *in, c64 *fourier, double *out) {
   extern __shared__ c64 smem[];
-  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_next[2];
   c64 *twl = smem + cfft_ub::kLdsPoints;
   uint32_t staged = 0xffffffffu;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t t = s_tile;
-    __syncthreads();
-    if (t >= ntiles) break;
+  if (threadIdx.x == 0) s_next[0] = atomicAdd(counter, 1u);
+  __syncthreads();
+  uint32_t t = s_next[0];
+  // The queue index of the next tile is taken when a tile starts and
+  // published when it ends, so the atomic's round trip overlaps the tile.
+  // Slot parity alternates: a slot is written one tile after all threads
+  // read it (a barrier lies between).
+  for (uint32_t slot = 1; t < ntiles; slot ^= 1u) {
+    uint32_t next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
     const UTile tile = tiles[t];
     const UClass &C = classes[tile.cls];
     if (tile.cls != staged) {
@@ -284,9 +289,12 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
       __syncthreads();
       staged = tile.cls;
     }
-    // opaque per tile: address arithmetic of the five bodies is not hoisted
+    // opaque per tile: address arithmetic of the six bodies is not hoisted
     // out of the loop (it would stay live across it and spill)
     run_tile_r16(C, tile, jobs, order, in, fourier, out, smem, twl, launder_u32(threadIdx.x));
+    if (threadIdx.x == 0) s_next[slot] = next;
+    __syncthreads();
+    t = s_next[slot];
   }
 }
 

@@ -70,9 +70,9 @@ def test_uneven_bit_exact(cfft, oracle, measure):
 
 
 def test_uneven_dif16_body_bit_exact(cfft, oracle):
-    # Dif16 plans: N = 1024 .. 16384 run the register-resident Dif16 body
+    # Dif16 plans: N = 512 .. 16384 run the register-resident Dif16 body
     # (wave-local for N <= 2048, one depth pass around 1024-point base
-    # transforms above), N <= 512 the generic body, both in one schedule;
+    # transforms above), N <= 256 the generic body, both in one schedule;
     # partial last tiles in every class
     from cfft_amd.device import DeviceBuffer, synchronize
     from cfft_amd.uneven import Schedule
@@ -81,7 +81,7 @@ def test_uneven_dif16_body_bit_exact(cfft, oracle):
     log2_N = rng.choice(np.arange(5, 15), size=300)
     jobs, nin, nf, nout = _jobs_with_gaps(log2_N, rng)
     sched = Schedule(jobs, nin, nf, nout, algo=FftAlgo.Dif16)
-    for l in range(10, 15):
+    for l in range(9, 15):
         assert sched.class_algo(l) == (FftAlgo.Dif16, min(1 << (l - 1), 1024))
     dx = DeviceBuffer((nin,), np.float64)
     dF = DeviceBuffer((nf,), np.complex128)
