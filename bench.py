@@ -131,13 +131,12 @@ def bench_uneven(steps, warmup):
     tuned = Schedule(jobs, nin, nf, nout, measure=True)
     tune_s = time.perf_counter() - t0
     fixed = Schedule(jobs, nin, nf, nout, measure=False, algo=FftAlgo.Dif4)
-    dif16 = Schedule(jobs, nin, nf, nout, measure=False, algo=FftAlgo.Dif16)
     forms = {
         "tuned_persistent_graph": lambda: tuned.run(dx, dF, dy, s),
+        "tuned_persistent_direct": lambda: tuned.run_direct(dx, dF, dy, s),
         "tuned_plain": lambda: tuned.run_plain(dx, dF, dy, s),
         "fixed_persistent_graph": lambda: fixed.run(dx, dF, dy, s),
         "fixed_plain": lambda: fixed.run_plain(dx, dF, dy, s),
-        "dif16_persistent_graph": lambda: dif16.run(dx, dF, dy, s),
     }
     ms = {k: float(np.median(time_runs(f, s, steps, warmup))) for k, f in forms.items()}
     tuned.run(dx, dF, dy, s)
@@ -145,7 +144,8 @@ def bench_uneven(steps, warmup):
     err = float(np.abs(dy.download() - x).max())
     total_N = int(np.sum(np.left_shift(1, log2_N.astype(np.int64))))
     bytes_algo = 24 * total_N  # x read, Fourier written, round trip written (8N each)
-    best = ms["tuned_persistent_graph"]
+    best_form = min(("tuned_persistent_graph", "tuned_persistent_direct"), key=lambda k: ms[k])
+    best = ms[best_form]
     plans = {}
     for l in UNEVEN_LOG2N:
         a, b = tuned.class_algo(l)
@@ -153,10 +153,11 @@ def bench_uneven(steps, warmup):
     return {
         "workload": "%d f64 negacyclic fwd+inv jobs, N in {512..16384} weights %s, %d total "
                     "coefficients" % (UNEVEN_JOBS, UNEVEN_WEIGHTS, total_N),
+        "form": best_form, "tile_schedule": tuned.tile_schedule(),
         "jobs_per_s": round(UNEVEN_JOBS / (best * 1e-3), 1),
         "transforms_per_s": round(2 * UNEVEN_JOBS / (best * 1e-3), 1),
         "ms": {k: round(v, 4) for k, v in ms.items()},
-        "tuned_graph_over_fixed_plain": round(ms["fixed_plain"] / best, 3),
+        "tuned_over_fixed_plain": round(ms["fixed_plain"] / best, 3),
         "roofline": {"bound": "hbm", "achieved": round(bytes_algo / (best * 1e-3) / 1e9, 2),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(bytes_algo / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),

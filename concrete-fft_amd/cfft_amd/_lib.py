@@ -109,6 +109,9 @@ _SIGS = [
     ("cfft_uneven_destroy", c_int, [c_vp]),
     ("cfft_uneven_run", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("cfft_uneven_run_plain", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("cfft_uneven_run_direct", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("cfft_uneven_tile_schedule", c_int, [c_vp, ctypes.POINTER(c_int)]),
+    ("cfft_uneven_set_tile_schedule", c_int, [c_vp, c_int]),
     ("cfft_uneven_class_algo", c_int, [c_vp, ctypes.c_uint32, ctypes.POINTER(c_int),
                                        ctypes.POINTER(c_size)]),
     ("cfft_server_create", c_int, [ctypes.POINTER(ServerConfig), ctypes.POINTER(c_vp)]),

@@ -35,8 +35,9 @@ def packed_jobs(log2_N):
 
 
 class Schedule:
-    """cfft_uneven: built once per job list; `run` replays one persistent
-    launch from a captured graph, `run_plain` launches once per size class."""
+    """cfft_uneven: built once per job list; `run` replays the persistent
+    launches from a captured graph, `run_direct` issues them on the stream,
+    `run_plain` launches once per size class."""
 
     def __init__(self, jobs, in_len, fourier_len, out_len, measure=False, algo=FftAlgo.Dif4,
                  seconds=0.0):
@@ -51,8 +52,22 @@ class Schedule:
     def run(self, x, fourier, y, stream=None):
         check(lib.cfft_uneven_run(self.handle, x.dptr, fourier.dptr, y.dptr, _h(stream)))
 
+    def run_direct(self, x, fourier, y, stream=None):
+        check(lib.cfft_uneven_run_direct(self.handle, x.dptr, fourier.dptr, y.dptr, _h(stream)))
+
     def run_plain(self, x, fourier, y, stream=None):
         check(lib.cfft_uneven_run_plain(self.handle, x.dptr, fourier.dptr, y.dptr, _h(stream)))
+
+    def tile_schedule(self):
+        """"device_queue" or "static_deal" (how the Dif16 tiles reach the
+        persistent workgroups)."""
+        k = ctypes.c_int()
+        check(lib.cfft_uneven_tile_schedule(self.handle, ctypes.byref(k)))
+        return "device_queue" if k.value == 0 else "static_deal"
+
+    def set_tile_schedule(self, kind):
+        check(lib.cfft_uneven_set_tile_schedule(
+            self.handle, {"device_queue": 0, "static_deal": 1}[kind]))
 
     def class_algo(self, log2_N):
         a, b = ctypes.c_int(), ctypes.c_size_t()

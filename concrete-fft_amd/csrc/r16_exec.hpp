@@ -259,7 +259,9 @@ __device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_
 
 // One transform of n = 2^LOG2N points (LOG2N = 8, 9 or 10) on T = n/16
 // lanes of one wavefront (64 / T transforms per wave, each on its own lanes).
-// `tw` is the r16 twiddle block of R16<LOG2N> (in LDS); `x` is the
+// `tw` is the r16 twiddle block of the 1024-point transform (in LDS): the
+// twiddles of a smaller transform are its entries at column stride 1024 / n
+// (T_n[j] = T_1024[j * 1024 / n], the same sincospi argument).  `x` is the
 // transform's own exchange buffer (R16<LOG2N>::lds_words points).  Same
 // layouts and operation order as r16_transform; no s_barrier.  At n = 256
 // the reference's Dif16 is one core at stride 1 and a radix-16 end at stride
@@ -268,9 +270,10 @@ template <int LOG2N, bool FWD>
 __device__ __forceinline__ void r16l_transform(c64 *v, c64 *x, uint32_t l, const c64 *tw) {
   using S = R16<LOG2N>;
   static_assert(S::T <= 64 && LOG2N >= 8, "one transform must fit a wavefront");
+  constexpr uint32_t cs = 1u << (10 - LOG2N);  // column stride in the 1024 block
   butterfly<16, FWD>(v);
   sched_fence();
-  twiddle16_x1<FWD>(v, tw, S::T, l);
+  twiddle16_x1<FWD>(v, tw, 64, l * cs);
   if constexpr (LOG2N == 8) {
     wave_lds_fence();
 #pragma unroll
@@ -291,7 +294,7 @@ __device__ __forceinline__ void r16l_transform(c64 *v, c64 *x, uint32_t l, const
   for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
   butterfly<16, FWD>(v);
   sched_fence();
-  twiddle16_x1<FWD>(v, tw + 16 * S::T, S::T / 16, p);
+  twiddle16_x1<FWD>(v, tw + 16 * 64, 4, p * cs);
   wave_lds_fence();
 #pragma unroll
   for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];

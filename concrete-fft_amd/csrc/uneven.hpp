@@ -14,10 +14,18 @@ constexpr uint32_t kThreads = 512;  // one workgroup: 8 waves
 // LDS of one workgroup: (8192 / n) transform groups of n + n/16 points
 constexpr uint32_t kLdsPoints = 8192 + 512;
 constexpr size_t kLdsBytes = kLdsPoints * 16;
-// Dif16 body: tile data followed by the class's base twiddle block
-// (rows k < 16 of T[k l], l < 64, then rows of T[16 k p], p < 4)
+// Dif16 body: tile data followed by the twiddle block of the 1024-point
+// base transform (rows k < 16 of T[k l], l < 64, then rows of T[16 k p],
+// p < 4); every Dif16 class reads its base twiddles from it (a class with
+// base_n < 1024 at column stride 1024 / base_n)
 constexpr uint32_t kTwR16Points = 16 * 64 + 16 * 4;
 constexpr size_t kR16LdsBytes = (size_t)(kLdsPoints + kTwR16Points) * 16;
+// device work queue of the Dif16 body: partitions (one counter each, on its
+// own 128-byte line) and the counter buffer's layout
+constexpr uint32_t kQueueParts = 8;
+constexpr uint32_t kCounterStride = 32;  // words
+constexpr uint32_t kGenericCounter = kQueueParts * kCounterStride;
+constexpr size_t kCounterBytes = (kGenericCounter + kCounterStride) * 4;
 constexpr uint32_t kMinLog2N = 5;   // N = 32: n = 16, one thread per transform
 constexpr uint32_t kMaxLog2N = 14;  // N = 16384: n = 8192, one transform per workgroup
 
@@ -42,10 +50,6 @@ struct UClass {
   const cfft_launch::dc64 *tw;  // T[j], n entries
   const cfft_launch::dc64 *twist;
   const cfft_launch::dc64 *untwist;
-  // Dif16 plans with base_n = min(n, 1024), 256 <= n <= 8192: twiddle block
-  // of the base transform (T[k l tw_mul] as [k][l], then T[16 k p tw_mul] as
-  // [k][p], tw_mul = n / base_n), null for classes run by the generic body
-  const cfft_launch::dc64 *tw_r16;
   uint32_t log2n;
   uint32_t r16;  // 1: the class runs the Dif16 body
 };
@@ -60,15 +64,25 @@ constexpr uint32_t groups_per_tile(uint32_t log2n) { return kThreads / ((1u << l
 
 // sets the kernels' LDS limit on the current device (before any launch)
 hipError_t prepare();
-// Resets `counter`, then runs tiles[0 .. ntiles) on `grid` persistent
-// workgroups; `r16` selects the Dif16 body (every tile's class must have r16
-// set) or the generic body.
-hipError_t launch_persistent(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
+// Generic body: resets counter[0], then runs tiles[0 .. ntiles) on `grid`
+// persistent workgroups.
+hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
                              uint32_t ntiles, const UClass *classes, uint32_t *counter,
                              uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
                              double *out, hipStream_t s);
-hipError_t launch_tiles(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
+// Dif16 body: runs tiles[0 .. ntiles) on `grid` persistent workgroups, fed
+// by the partitioned device queue at `counter` (kQueueParts counters, reset
+// by a kernel launched first) or, with counter null, dealt round-robin.
+hipError_t launch_r16_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+                                 uint32_t ntiles, const UClass *classes,
+                                 const cfft_launch::dc64 *tw_block, uint32_t *counter,
+                                 uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
+                                 double *out, hipStream_t s);
+// Plain form: one workgroup per tile; `tw_block` non-null selects the Dif16
+// body (every tile's class must be a Dif16 class).
+hipError_t launch_tiles(const UJob *jobs, const uint32_t *order, const UTile *tiles,
                         uint32_t first_tile, uint32_t count, const UClass *classes,
-                        const double *in, cfft_launch::dc64 *fourier, double *out, hipStream_t s);
+                        const cfft_launch::dc64 *tw_block, const double *in,
+                        cfft_launch::dc64 *fourier, double *out, hipStream_t s);
 
 }  // namespace cfft_ub

@@ -8,13 +8,17 @@
 // the fastest of a candidate set, timed on the device with this schedule's
 // own kernels), tiles of up to 8192/n jobs of one class, queued largest class
 // first so the long tiles start early and the short ones fill the tail.
-// Classes whose plan is Dif16 with base_n = min(n, 1024) (512 <= n <= 8192)
-// run the register-resident Dif16 body in their own persistent launch (their
-// tiles come first in the queue); the others run the generic body.
+// Classes whose plan is Dif16 with base_n = min(n, 1024) (256 <= n <= 8192)
+// run the register-resident Dif16 body in their own persistent launch, fed
+// by a device tile queue or dealt statically (measured: the queue's one
+// device-scope atomic per tile is a serialised round trip); the other
+// classes run the generic body in a second persistent launch fed by a
+// device tile queue.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -37,10 +41,10 @@ struct DevSchedule {
   uint32_t *order = nullptr;
   UTile *tiles = nullptr;
   UClass *classes = nullptr;
-  dc64 *tw_r16 = nullptr;      // base twiddle blocks of the Dif16 classes
-  uint32_t *counter = nullptr;  // [0]: Dif16 queue, [32]: generic queue
+  dc64 *tw_r16 = nullptr;       // twiddle block of the 1024-point base transform
+  uint32_t *counter = nullptr;  // tile queues: Dif16 partitions, then the generic body
   uint32_t ntiles = 0;
-  uint32_t ntiles_r16 = 0;      // tiles [0, ntiles_r16) run the Dif16 body
+  uint32_t ntiles_r16 = 0;      // tiles [0, ntiles_r16) are Dif16 classes
   void release() {
     if (jobs) (void)hipFree(jobs);
     if (order) (void)hipFree(order);
@@ -110,43 +114,54 @@ HostSchedule make_schedule(const std::vector<std::vector<uint32_t>> &jobs_by_cla
   return h;
 }
 
-// base twiddle block of a Dif16 class (layout of UClass::tw_r16)
-std::vector<cfft_host::C64> r16_block(const cfft_plan_s *p) {
-  const size_t n = p->n, base = p->base_n, mul = n / base;
-  const size_t T = base / 16, T2 = T / 16;
+// twiddle block of the Dif16 base transform of a plan of size n (rows
+// k < 16 of T[k l tw_mul], l < base / 16, then rows of T[16 k p tw_mul],
+// p < base / 256; tw_mul = n / base)
+std::vector<cfft_host::C64> r16_block(size_t n, size_t base) {
+  const size_t mul = n / base;
+  const size_t T = base / 16, T2 = std::max<size_t>(T / 16, 1);
   const std::vector<cfft_host::C64> tw = cfft_host::twiddle_table(n);
   std::vector<cfft_host::C64> blk(16 * T + 16 * T2);
   for (size_t k = 0; k < 16; ++k) {
     for (size_t l = 0; l < T; ++l) blk[k * T + l] = tw[k * l * mul];
-    for (size_t q = 0; q < T2; ++q) blk[16 * T + k * T2 + q] = tw[16 * k * q * mul];
+    for (size_t q = 0; q < T2; ++q) blk[16 * T + k * T2 + q] = tw[(16 * k * q * mul) % n];
   }
   return blk;
 }
 
-// `plans[c]` is the plan of classes[c]; fills in the Dif16 twiddle blocks
-hipError_t upload_schedule(DevSchedule &d, const std::vector<UJob> &jobs, const HostSchedule &h,
-                           std::vector<UClass> classes,
-                           const std::vector<const cfft_plan_s *> &plans) {
-  std::vector<cfft_host::C64> blocks;
-  std::vector<size_t> at(classes.size(), 0);
-  for (size_t c = 0; c < classes.size(); ++c) {
-    if (!classes[c].r16) continue;
-    at[c] = blocks.size();
-    const std::vector<cfft_host::C64> b = r16_block(plans[c]);
-    blocks.insert(blocks.end(), b.begin(), b.end());
+bool same_bits(const cfft_host::C64 &a, const cfft_host::C64 &b) {
+  return std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+// The Dif16 body reads every class's base twiddles from the 1024-point
+// block at column stride 1024 / base_n; true if the plan's own table agrees
+// bit for bit (it does whenever both come from sincospi of the exact
+// argument -2 j / n).
+bool block_covers(const std::vector<cfft_host::C64> &b1024, const cfft_plan_s *p) {
+  const std::vector<cfft_host::C64> own = r16_block(p->n, p->base_n);
+  const size_t T = p->base_n / 16, T2 = T / 16, cs = 1024 / p->base_n;
+  for (size_t k = 0; k < 16; ++k) {
+    for (size_t l = 0; l < T; ++l)
+      if (!same_bits(own[k * T + l], b1024[k * 64 + l * cs])) return false;
+    for (size_t q = 0; q < T2; ++q)
+      if (!same_bits(own[16 * T + k * T2 + q], b1024[1024 + k * 4 + q * cs])) return false;
   }
+  return true;
+}
+
+hipError_t upload_schedule(DevSchedule &d, const std::vector<UJob> &jobs, const HostSchedule &h,
+                           const std::vector<UClass> &classes) {
   hipError_t e = upload(&d.jobs, jobs);
   if (e == hipSuccess) e = upload(&d.order, h.order);
   if (e == hipSuccess) e = upload(&d.tiles, h.tiles);
-  if (e == hipSuccess && !blocks.empty()) {
-    e = hipMalloc(reinterpret_cast<void **>(&d.tw_r16), blocks.size() * sizeof(dc64));
+  if (e == hipSuccess && h.ntiles_r16) {
+    const std::vector<cfft_host::C64> blk = r16_block(1024, 1024);
+    e = hipMalloc(reinterpret_cast<void **>(&d.tw_r16), blk.size() * sizeof(dc64));
     if (e == hipSuccess)
-      e = hipMemcpy(d.tw_r16, blocks.data(), blocks.size() * sizeof(dc64), hipMemcpyHostToDevice);
-    for (size_t c = 0; c < classes.size(); ++c)
-      if (classes[c].r16) classes[c].tw_r16 = d.tw_r16 + at[c];
+      e = hipMemcpy(d.tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice);
   }
   if (e == hipSuccess) e = upload(&d.classes, classes);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.counter), 256);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.counter), cfft_ub::kCounterBytes);
   d.ntiles = (uint32_t)h.tiles.size();
   d.ntiles_r16 = h.ntiles_r16;
   return e;
@@ -163,22 +178,48 @@ UClass class_of(const cfft_plan_s *p) {
   while (((size_t)1 << l) < p->n) ++l;
   while (((size_t)1 << lb) < p->base_n) ++lb;
   c.log2n = l;
-  c.tw_r16 = nullptr;
-  c.r16 = cfft_ub::r16_class(p->algo == CFFT_DIF16, l, lb) ? 1u : 0u;
+  c.r16 = 0;
+  if (cfft_ub::r16_class(p->algo == CFFT_DIF16, l, lb))
+    c.r16 = block_covers(r16_block(1024, 1024), p) ? 1u : 0u;
   return c;
 }
 
-// the Dif16 tiles, then the generic tiles, each on its own queue counter
-hipError_t launch_schedule(const DevSchedule &d, uint32_t grid, const double *in, dc64 *f,
-                           double *out, hipStream_t s) {
+// the Dif16 classes (device queue or static deal), then the generic ones
+// (device queue), each launch on its own queue counter
+hipError_t launch_schedule(const DevSchedule &d, uint32_t grid, bool r16_queue, const double *in,
+                           dc64 *f, double *out, hipStream_t s) {
   hipError_t e = hipSuccess;
   if (d.ntiles_r16)
-    e = cfft_ub::launch_persistent(true, d.jobs, d.order, d.tiles, d.ntiles_r16, d.classes,
-                                   d.counter, grid, in, f, out, s);
+    e = cfft_ub::launch_r16_persistent(d.jobs, d.order, d.tiles, d.ntiles_r16, d.classes,
+                                       d.tw_r16, r16_queue ? d.counter : nullptr, grid, in, f,
+                                       out, s);
   if (e == hipSuccess && d.ntiles > d.ntiles_r16)
-    e = cfft_ub::launch_persistent(false, d.jobs, d.order, d.tiles + d.ntiles_r16,
-                                   d.ntiles - d.ntiles_r16, d.classes, d.counter + 32, grid, in,
+    e = cfft_ub::launch_persistent(d.jobs, d.order, d.tiles + d.ntiles_r16,
+                                   d.ntiles - d.ntiles_r16, d.classes,
+                                   d.counter + cfft_ub::kGenericCounter, grid, in,
                                    f, out, s);
+  return e;
+}
+
+// best of 3 timed direct launches (after one warm-up) on the given buffers
+hipError_t time_schedule(const DevSchedule &d, uint32_t grid, bool r16_queue, const double *x,
+                         dc64 *f, double *y, double *ms) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double best = 1e30;
+  for (int it = 0; it < 4 && e == hipSuccess; ++it) {
+    e = hipEventRecord(e0, nullptr);
+    if (e == hipSuccess) e = launch_schedule(d, grid, r16_queue, x, f, y, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess && it > 0) best = std::min(best, (double)t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  *ms = best;
   return e;
 }
 
@@ -223,6 +264,7 @@ struct cfft_uneven_s {
   HostSchedule host;
   DevSchedule dev;
   uint32_t grid = 0;
+  bool r16_queue = true;  // Dif16 tiles: device queue (true) or static deal
   std::mutex mu;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
@@ -256,26 +298,15 @@ int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t gr
   DevSchedule d;
   double *x = nullptr, *y = nullptr;
   dc64 *f = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  hipError_t e = upload_schedule(d, jobs, h, cls, {p});
+  hipError_t e = upload_schedule(d, jobs, h, cls);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x), 2 * n * njobs * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&y), 2 * n * njobs * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&f), n * njobs * sizeof(dc64));
   if (e == hipSuccess) e = hipMemset(x, 0, 2 * n * njobs * sizeof(double));
-  if (e == hipSuccess) e = hipEventCreate(&e0);
-  if (e == hipSuccess) e = hipEventCreate(&e1);
-  double best = 1e30;
-  for (int it = 0; it < 4 && e == hipSuccess; ++it) {
-    e = hipEventRecord(e0, nullptr);
-    if (e == hipSuccess) e = launch_schedule(d, grid, x, f, y, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    float t = 0.f;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-    if (e == hipSuccess && it > 0) best = std::min(best, (double)t);
-  }
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
+  double best = 1e30, t_static = 1e30;
+  if (e == hipSuccess) e = time_schedule(d, grid, true, x, f, y, &best);
+  if (e == hipSuccess && d.ntiles_r16) e = time_schedule(d, grid, false, x, f, y, &t_static);
+  best = std::min(best, t_static);
   if (x) (void)hipFree(x);
   if (y) (void)hipFree(y);
   if (f) (void)hipFree(f);
@@ -302,6 +333,34 @@ int check_job(const cfft_uneven_job &j, size_t idx, size_t in_len, size_t f_len,
   return CFFT_SUCCESS;
 }
 
+// Method::Measure: times the Dif16 launch with the device queue and with the
+// static deal on zeroed buffers of the schedule's own sizes (skipped, keeping
+// the queue, when they would take more than a quarter of free device memory)
+int choose_tile_schedule(cfft_uneven_s *u) {
+  size_t free_b = 0, total_b = 0;
+  hipError_t e = hipMemGetInfo(&free_b, &total_b);
+  if (e != hipSuccess) return hip_error(e, "device query");
+  const size_t need = (u->in_len + u->out_len) * sizeof(double) + u->f_len * sizeof(dc64);
+  if (need > free_b / 4) return CFFT_SUCCESS;
+  double *x = nullptr, *y = nullptr;
+  dc64 *f = nullptr;
+  e = hipMalloc(reinterpret_cast<void **>(&x), std::max<size_t>(u->in_len, 1) * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&y), std::max<size_t>(u->out_len, 1) * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&f), std::max<size_t>(u->f_len, 1) * sizeof(dc64));
+  if (e == hipSuccess) e = hipMemset(x, 0, std::max<size_t>(u->in_len, 1) * sizeof(double));
+  double t_queue = 0, t_static = 0;
+  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, true, x, f, y, &t_queue);
+  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, false, x, f, y, &t_static);
+  if (x) (void)hipFree(x);
+  if (y) (void)hipFree(y);
+  if (f) (void)hipFree(f);
+  if (e != hipSuccess) return hip_error(e, "uneven autotuner");
+  u->r16_queue = t_queue <= t_static;
+  return CFFT_SUCCESS;
+}
+
 int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int method, int base_algo,
           double seconds) {
   (void)seconds;
@@ -318,7 +377,6 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
   if (ep != hipSuccess) return hip_error(ep, "uneven kernel attributes");
   std::vector<std::vector<uint32_t>> by_class;
   std::vector<UClass> classes;
-  std::vector<const cfft_plan_s *> class_plans;
   for (uint32_t l = 0; l < by_log2n.size(); ++l) {
     if (by_log2n[l].empty()) continue;
     const size_t n = (size_t)1 << l;
@@ -344,14 +402,15 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
     u->plans.push_back(p);
     u->class_log2n.push_back(l);
     classes.push_back(class_of(p));
-    class_plans.push_back(p);
     by_class.push_back(by_log2n[l]);
   }
   u->host = make_schedule(by_class, classes);
-  hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes, class_plans);
+  hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  return hip_error(e, "uneven schedule upload");
+  if (e != hipSuccess) return hip_error(e, "uneven schedule upload");
+  if (method == CFFT_METHOD_MEASURE && u->dev.ntiles_r16) return choose_tile_schedule(u);
+  return CFFT_SUCCESS;
 }
 
 }  // namespace
@@ -427,8 +486,8 @@ int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *ou
     u->graph = nullptr;
     e = hipStreamBeginCapture(u->cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipError_t el =
-          launch_schedule(u->dev, u->grid, in, reinterpret_cast<dc64 *>(fourier), out, u->cap);
+      hipError_t el = launch_schedule(u->dev, u->grid, u->r16_queue, in,
+                                      reinterpret_cast<dc64 *>(fourier), out, u->cap);
       e = hipStreamEndCapture(u->cap, &u->graph);
       if (el != hipSuccess) e = el;
     }
@@ -441,14 +500,46 @@ int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *ou
   return hip_error(hipGraphLaunch(u->exec, (hipStream_t)stream), "uneven graph launch");
 }
 
+int cfft_uneven_run_direct(cfft_uneven u, const double *in, double *fourier, double *out,
+                           cfft_stream_t stream) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (u->jobs.empty()) return CFFT_SUCCESS;
+  if (!in || !fourier || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  std::lock_guard<std::mutex> g(u->mu);  // the launches share the queue counters
+  return hip_error(launch_schedule(u->dev, u->grid, u->r16_queue, in,
+                                   reinterpret_cast<dc64 *>(fourier), out, (hipStream_t)stream),
+                   "uneven launch");
+}
+
+int cfft_uneven_tile_schedule(cfft_uneven u, int *kind) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (!kind) return set_error(CFFT_ERROR_NULL_POINTER, "kind is null");
+  *kind = u->r16_queue ? CFFT_UNEVEN_DEVICE_QUEUE : CFFT_UNEVEN_STATIC_DEAL;
+  return CFFT_SUCCESS;
+}
+
+int cfft_uneven_set_tile_schedule(cfft_uneven u, int kind) {
+  if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
+  if (kind != CFFT_UNEVEN_DEVICE_QUEUE && kind != CFFT_UNEVEN_STATIC_DEAL)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown tile schedule");
+  std::lock_guard<std::mutex> g(u->mu);
+  u->r16_queue = kind == CFFT_UNEVEN_DEVICE_QUEUE;
+  if (u->exec) (void)hipGraphExecDestroy(u->exec);  // captured with the old choice
+  if (u->graph) (void)hipGraphDestroy(u->graph);
+  u->exec = nullptr;
+  u->graph = nullptr;
+  return CFFT_SUCCESS;
+}
+
 int cfft_uneven_run_plain(cfft_uneven u, const double *in, double *fourier, double *out,
                           cfft_stream_t stream) {
   if (!u) return set_error(CFFT_ERROR_NULL_POINTER, "schedule is null");
   if (u->jobs.empty()) return CFFT_SUCCESS;
   if (!in || !fourier || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   for (const ClassRange &r : u->host.ranges) {
-    hipError_t e = cfft_ub::launch_tiles(r.r16, u->dev.jobs, u->dev.order, u->dev.tiles,
-                                         r.first_tile, r.ntiles, u->dev.classes, in,
+    hipError_t e = cfft_ub::launch_tiles(u->dev.jobs, u->dev.order, u->dev.tiles, r.first_tile,
+                                         r.ntiles, u->dev.classes,
+                                         r.r16 ? u->dev.tw_r16 : nullptr, in,
                                          reinterpret_cast<dc64 *>(fourier), out,
                                          (hipStream_t)stream);
     if (e != hipSuccess) return hip_error(e, "uneven plain launch");

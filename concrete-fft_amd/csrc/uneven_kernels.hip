@@ -21,8 +21,9 @@
 //    transform.  For n > 1024 the plan's single depth pass (radix n / 1024,
 //    src/unordered.rs:391-489) runs in registers right after the loads
 //    (forward) and right before the stores (inverse), so a tile needs two
-//    workgroup barriers; for n <= 1024 a tile is wave-local.  The base
-//    transform's twiddle block is staged in LDS once per class change.
+//    workgroup barriers; jobs with n <= 1024 are wave-local (no barrier
+//    inside a tile).  One twiddle block (the 1024-point base transform's)
+//    serves every class and is staged in LDS once per workgroup.
 #include <hip/hip_runtime.h>
 
 #include "launch.hpp"
@@ -100,20 +101,19 @@ __device__ __forceinline__ void run_tile(const UClass &C, const UTile &tile,
 
 // ---- Dif16 body ----
 
-// n = 2^LOG2N, 256 <= n <= 1024: one job on T = n/16 lanes of one wavefront
+// One job of n = 2^LOG2N points (256 <= n <= 1024) on T = n/16 lanes of one
+// wavefront: fold, twist, forward, Fourier store, inverse, untwist, store.
+// `job` is null on lanes whose group has no job; `x` is the group's exchange
+// buffer, `twl` the 1024-point twiddle block in LDS.
 template <int LOG2N>
-__device__ __forceinline__ void tile_r16_local(const UClass &C, const UTile &tile,
-                                               const UJob *__restrict__ jobs,
-                                               const uint32_t *__restrict__ order,
-                                               const double *in, c64 *fourier, double *out,
-                                               c64 *smem, const c64 *twl, uint32_t tid) {
+__device__ __forceinline__ void job_r16_local(const UClass &C, const UJob *job, const double *in,
+                                              c64 *fourier, double *out, c64 *x, uint32_t l,
+                                              const c64 *twl) {
   using S = R16<LOG2N>;
   constexpr uint32_t n = S::n, T = S::T;
-  const uint32_t g = tid / T, l = tid % T;
-  const bool active = g < tile.count;
-  c64 *x = smem + g * S::lds_words;
+  const bool active = job != nullptr;
   JobPtrs P{nullptr, nullptr, nullptr};
-  if (active) P = job_ptrs(jobs[order[tile.first + g]], in, fourier, out);
+  if (active) P = job_ptrs(*job, in, fourier, out);
   const c64 *twist = reinterpret_cast<const c64 *>(C.twist);
   const c64 *untwist = reinterpret_cast<const c64 *>(C.untwist);
   c64 v[16];
@@ -138,6 +138,32 @@ __device__ __forceinline__ void tile_r16_local(const UClass &C, const UTile &til
       st_stream(r.re, &P.y[j]);
       st_stream(r.im, &P.y[j + n]);
     }
+  }
+}
+
+// jobs tile.first .. + tile.count on groups of T lanes; group g uses the
+// exchange buffer at g * lds_words (so wave w's groups share w * 1088)
+template <int LOG2N>
+__device__ __forceinline__ void group_r16_local(const UClass &C, const UTile &tile,
+                                                const UJob *__restrict__ jobs,
+                                                const uint32_t *__restrict__ order,
+                                                const double *in, c64 *fourier, double *out,
+                                                c64 *xbase, const c64 *twl, uint32_t tid) {
+  using S = R16<LOG2N>;
+  const uint32_t g = tid / S::T, l = tid % S::T;
+  const UJob *job = g < tile.count ? &jobs[order[tile.first + g]] : nullptr;
+  job_r16_local<LOG2N>(C, job, in, fourier, out, xbase + g * S::lds_words, l, twl);
+}
+
+__device__ __forceinline__ void run_group_r16_local(const UClass &C, const UTile &tile,
+                                                    const UJob *__restrict__ jobs,
+                                                    const uint32_t *__restrict__ order,
+                                                    const double *in, c64 *fourier, double *out,
+                                                    c64 *xbase, const c64 *twl, uint32_t tid) {
+  switch (C.log2n) {
+    case 8: group_r16_local<8>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
+    case 9: group_r16_local<9>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
+    default: group_r16_local<10>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
   }
 }
 
@@ -241,74 +267,85 @@ __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &til
   }
 }
 
-__device__ __forceinline__ void run_tile_r16(const UClass &C, const UTile &tile,
-                                             const UJob *__restrict__ jobs,
-                                             const uint32_t *__restrict__ order,
-                                             const double *in, c64 *fourier, double *out,
-                                             c64 *smem, const c64 *twl, uint32_t tid) {
+__device__ __forceinline__ void run_tile_r16_depth(const UClass &C, const UTile &tile,
+                                                   const UJob *__restrict__ jobs,
+                                                   const uint32_t *__restrict__ order,
+                                                   const double *in, c64 *fourier, double *out,
+                                                   c64 *smem, const c64 *twl, uint32_t tid) {
   switch (C.log2n) {
-    case 8: tile_r16_local<8>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
-    case 9: tile_r16_local<9>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
-    case 10: tile_r16_local<10>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 11: tile_r16_depth<11>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 12: tile_r16_depth<12>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     default: tile_r16_depth<13>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
   }
 }
 
-// copies the class's base twiddle block into LDS (callers barrier after)
-__device__ __forceinline__ void stage_r16_twiddles(const UClass &C, c64 *twl) {
-  const uint32_t T = C.log2n >= 10 ? 64u : (1u << C.log2n) / 16;
-  const uint32_t words = 16 * T + T;
-  const c64 *src = reinterpret_cast<const c64 *>(C.tw_r16);
-  for (uint32_t i = threadIdx.x; i < words; i += kThreads) twl[i] = src[i];
+__device__ __forceinline__ void stage_r16_twiddles(const c64 *__restrict__ src, c64 *twl) {
+  for (uint32_t i = threadIdx.x; i < cfft_ub::kTwR16Points; i += kThreads) twl[i] = src[i];
 }
 
+// Tiles of the Dif16 classes on persistent workgroups.  With `counter`
+// (zeroed before the launch) the tiles come from a device work queue split
+// into P = min(kQueueParts, grid) partitions: partition p holds the tiles
+// t = p + P i, has its own counter (counter[p * kCounterStride], each on its
+// own cache line) and serves the workgroups w with w % P == p (workgroups
+// are dealt to the XCDs round-robin, so a partition's fetches stay on one
+// XCD).  A workgroup takes its next index when a tile starts and publishes
+// it when the tile ends, so the atomic's round trip overlaps the tile.
+// Without `counter` the tiles are dealt round-robin (workgroup w: tiles w,
+// w + G, ...).  Device-scope atomics on one address are serialised (measured
+// ~50 ns each); the partitions divide that cost, the static deal removes it.
 __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
     const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
     const UTile *__restrict__ tiles, uint32_t ntiles, const UClass *__restrict__ classes,
-    uint32_t *counter, const double *in, c64 *fourier, double *out) {
+    const c64 *__restrict__ tw_block, uint32_t *counter, const double *in, c64 *fourier,
+    double *out) {
+  // every partition needs a workgroup: P = min(kQueueParts, grid)
+  const uint32_t P = gridDim.x < cfft_ub::kQueueParts ? gridDim.x : cfft_ub::kQueueParts;
   extern __shared__ c64 smem[];
   __shared__ uint32_t s_next[2];
   c64 *twl = smem + cfft_ub::kLdsPoints;
-  uint32_t staged = 0xffffffffu;
-  if (threadIdx.x == 0) s_next[0] = atomicAdd(counter, 1u);
+  const bool queue = counter != nullptr;
+  const uint32_t part = blockIdx.x % P;
+  uint32_t *ctr = queue ? counter + part * cfft_ub::kCounterStride : nullptr;
+  // queue index i of partition `part` -> tile part + P i
+  stage_r16_twiddles(tw_block, twl);
+  if (queue && threadIdx.x == 0) s_next[0] = part + P * atomicAdd(ctr, 1u);
   __syncthreads();
-  uint32_t t = s_next[0];
-  // The queue index of the next tile is taken when a tile starts and
-  // published when it ends, so the atomic's round trip overlaps the tile.
-  // Slot parity alternates: a slot is written one tile after all threads
-  // read it (a barrier lies between).
+  uint32_t t = queue ? s_next[0] : blockIdx.x;
+  // queue slots: a slot is written one tile after every thread read it
   for (uint32_t slot = 1; t < ntiles; slot ^= 1u) {
     uint32_t next = 0;
-    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    if (queue && threadIdx.x == 0) next = part + P * atomicAdd(ctr, 1u);
     const UTile tile = tiles[t];
     const UClass &C = classes[tile.cls];
-    if (tile.cls != staged) {
-      stage_r16_twiddles(C, twl);
-      __syncthreads();
-      staged = tile.cls;
-    }
-    // opaque per tile: address arithmetic of the six bodies is not hoisted
-    // out of the loop (it would stay live across it and spill)
-    run_tile_r16(C, tile, jobs, order, in, fourier, out, smem, twl, launder_u32(threadIdx.x));
-    if (threadIdx.x == 0) s_next[slot] = next;
+    // opaque per tile: address arithmetic of the bodies is not hoisted out
+    // of the loop (it would stay live across it and spill)
+    const uint32_t tid = launder_u32(threadIdx.x);
+    if (C.log2n <= 10)
+      run_group_r16_local(C, tile, jobs, order, in, fourier, out, smem, twl, tid);
+    else
+      run_tile_r16_depth(C, tile, jobs, order, in, fourier, out, smem, twl, tid);
+    if (queue && threadIdx.x == 0) s_next[slot] = next;
     __syncthreads();
-    t = s_next[slot];
+    t = queue ? s_next[slot] : t + gridDim.x;
   }
 }
 
+// plain form: one workgroup per tile of one class
 __global__ void __launch_bounds__(kThreads) k_uneven_r16_tiles(
     const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
     const UTile *__restrict__ tiles, uint32_t first_tile, const UClass *__restrict__ classes,
-    const double *in, c64 *fourier, double *out) {
+    const c64 *__restrict__ tw_block, const double *in, c64 *fourier, double *out) {
   extern __shared__ c64 smem[];
   c64 *twl = smem + cfft_ub::kLdsPoints;
   const UTile tile = tiles[first_tile + blockIdx.x];
   const UClass &C = classes[tile.cls];
-  stage_r16_twiddles(C, twl);
+  stage_r16_twiddles(tw_block, twl);
   __syncthreads();
-  run_tile_r16(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
+  if (C.log2n <= 10)
+    run_group_r16_local(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
+  else
+    run_tile_r16_depth(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
 }
 
 __global__ void __launch_bounds__(kThreads) k_uneven_persistent(
@@ -337,8 +374,9 @@ __global__ void __launch_bounds__(kThreads) k_uneven_tiles(
   run_tile(classes[tile.cls], tile, jobs, order, in, fourier, out, smem);
 }
 
-__global__ void k_reset_counter(uint32_t *counter) {
-  if (threadIdx.x == 0) counter[0] = 0u;
+// zeroes `count` queue counters kCounterStride words apart
+__global__ void k_reset_counter(uint32_t *counter, uint32_t count) {
+  if (threadIdx.x < count) counter[threadIdx.x * cfft_ub::kCounterStride] = 0u;
 }
 
 }  // namespace
@@ -358,33 +396,46 @@ hipError_t prepare() {
   return e;
 }
 
-hipError_t launch_persistent(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
+hipError_t launch_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
                              uint32_t ntiles, const UClass *classes, uint32_t *counter,
                              uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
                              double *out, hipStream_t s) {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter, 1u);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (ntiles == 0) return hipSuccess;
-  c64 *f = reinterpret_cast<c64 *>(fourier);
-  if (r16)
-    hipLaunchKernelGGL(k_uneven_r16_persistent, dim3(grid), dim3(kThreads), kR16LdsBytes, s, jobs,
-                       order, tiles, ntiles, classes, counter, in, f, out);
-  else
-    hipLaunchKernelGGL(k_uneven_persistent, dim3(grid), dim3(kThreads), kLdsBytes, s, jobs, order,
-                       tiles, ntiles, classes, counter, in, f, out);
+  hipLaunchKernelGGL(k_uneven_persistent, dim3(grid), dim3(kThreads), kLdsBytes, s, jobs, order,
+                     tiles, ntiles, classes, counter, in, reinterpret_cast<c64 *>(fourier), out);
   return hipGetLastError();
 }
 
-hipError_t launch_tiles(bool r16, const UJob *jobs, const uint32_t *order, const UTile *tiles,
+hipError_t launch_r16_persistent(const UJob *jobs, const uint32_t *order, const UTile *tiles,
+                                 uint32_t ntiles, const UClass *classes,
+                                 const cfft_launch::dc64 *tw_block, uint32_t *counter,
+                                 uint32_t grid, const double *in, cfft_launch::dc64 *fourier,
+                                 double *out, hipStream_t s) {
+  if (counter) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter, kQueueParts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_uneven_r16_persistent, dim3(grid), dim3(kThreads), kR16LdsBytes, s, jobs,
+                     order, tiles, ntiles, classes, reinterpret_cast<const c64 *>(tw_block),
+                     counter, in, reinterpret_cast<c64 *>(fourier), out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tiles(const UJob *jobs, const uint32_t *order, const UTile *tiles,
                         uint32_t first_tile, uint32_t count, const UClass *classes,
-                        const double *in, cfft_launch::dc64 *fourier, double *out,
-                        hipStream_t s) {
+                        const cfft_launch::dc64 *tw_block, const double *in,
+                        cfft_launch::dc64 *fourier, double *out, hipStream_t s) {
   if (count == 0) return hipSuccess;
   c64 *f = reinterpret_cast<c64 *>(fourier);
-  if (r16)
+  if (tw_block)
     hipLaunchKernelGGL(k_uneven_r16_tiles, dim3(count), dim3(kThreads), kR16LdsBytes, s, jobs,
-                       order, tiles, first_tile, classes, in, f, out);
+                       order, tiles, first_tile, classes, reinterpret_cast<const c64 *>(tw_block),
+                       in, f, out);
   else
     hipLaunchKernelGGL(k_uneven_tiles, dim3(count), dim3(kThreads), kLdsBytes, s, jobs, order,
                        tiles, first_tile, classes, in, f, out);

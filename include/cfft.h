@@ -255,18 +255,37 @@ typedef struct cfft_uneven_s *cfft_uneven;
 /* Builds a schedule for a job list (host array) over buffers of the stated
  * lengths (in and out in doubles, fourier in c64; every job is checked to lie
  * inside them).  method USER_PROVIDED: every size class uses base_algo with
- * base_n = min(n, 1024); MEASURE: each class uses the fastest of a candidate
- * set of plans, timed on the device (the choice replaces Method::Measure,
- * src/unordered.rs:553-640, for this workload).  Blocks until built. */
+ * base_n = min(n, 1024), and the Dif16 tiles run from the device queue;
+ * MEASURE: each class uses the fastest of a candidate set of plans, timed on
+ * the device (the choice replaces Method::Measure, src/unordered.rs:553-640,
+ * for this workload), then the device queue and the static deal of the
+ * Dif16 tiles are timed on the whole schedule (on temporary buffers of the
+ * stated lengths, when they fit in a quarter of free device memory) and the
+ * faster is kept.  Blocks until built. */
 int cfft_uneven_create(const cfft_uneven_job *jobs, size_t njobs, size_t in_len,
                        size_t fourier_len, size_t out_len, int method, int base_algo,
                        double measure_seconds, cfft_uneven *out);
 int cfft_uneven_destroy(cfft_uneven u);
-/* One persistent launch over a device tile queue, replayed from a HIP graph
- * captured for these buffer addresses (re-captured when they change).  Runs
- * of one schedule must not overlap (they share its queue counter). */
+/* Size classes whose plan is Dif16 with base_n = min(n, 1024) (N = 512 ..
+ * 16384) run in one persistent launch of the register-resident Dif16 tile
+ * body; the other classes in a second persistent launch of the generic body
+ * over a device tile queue.  Dif16 tiles are taken from a device queue or
+ * dealt round-robin to the workgroups (cfft_uneven_tile_schedule). */
+#define CFFT_UNEVEN_DEVICE_QUEUE 0
+#define CFFT_UNEVEN_STATIC_DEAL 1
+/* The persistent launches replayed from a HIP graph captured for these
+ * buffer addresses (re-captured when they change).  Runs of one schedule
+ * must not overlap (they share its queue counters). */
 int cfft_uneven_run(cfft_uneven u, const double *in, double *fourier, double *out,
                     cfft_stream_t stream);
+/* The same launches issued directly on the stream (no graph; a caller may
+ * capture them in its own graph: no allocation, copy or memset). */
+int cfft_uneven_run_direct(cfft_uneven u, const double *in, double *fourier, double *out,
+                           cfft_stream_t stream);
+/* CFFT_UNEVEN_DEVICE_QUEUE or CFFT_UNEVEN_STATIC_DEAL: query, or override
+ * the choice made at creation (results are the same bits either way). */
+int cfft_uneven_tile_schedule(cfft_uneven u, int *kind);
+int cfft_uneven_set_tile_schedule(cfft_uneven u, int kind);
 /* The plain form: one launch per size class, one workgroup per tile. */
 int cfft_uneven_run_plain(cfft_uneven u, const double *in, double *fourier, double *out,
                           cfft_stream_t stream);

@@ -29,6 +29,22 @@ def _jobs_with_gaps(log2_N, rng):
     return jobs, pin, pf, pout
 
 
+def _written(jobs, nf, nout):
+    """Masks of the Fourier and output positions the jobs write (the gaps
+    between jobs are never written, so fresh buffers differ there)."""
+    mf = np.zeros(nf, dtype=bool)
+    my = np.zeros(nout, dtype=bool)
+    for j in jobs:
+        N = 1 << int(j["log2_N"])
+        mf[int(j["fourier_offset"]):int(j["fourier_offset"]) + N // 2] = True
+        my[int(j["out_offset"]):int(j["out_offset"]) + N] = True
+    return mf, my
+
+
+def _same_written(a, b, mask):
+    return np.array_equal(a[mask].view(np.uint64), b[mask].view(np.uint64))
+
+
 def _check_against_oracle(oracle, sched, jobs, x, F, y):
     for l in np.unique(jobs["log2_N"]):
         algo, base_n = sched.class_algo(int(l))
@@ -62,11 +78,12 @@ def test_uneven_bit_exact(cfft, oracle, measure):
     F, y = dF.download(), dy.download()
     _check_against_oracle(oracle, sched, jobs, x, F, y)
     # plain form: same bits
+    mf, my = _written(jobs, nf, nout)
     dF2 = DeviceBuffer((nf,), np.complex128)
     dy2 = DeviceBuffer((nout,), np.float64)
     sched.run_plain(dx, dF2, dy2)
-    assert np.array_equal(dF2.download().view(np.uint64), F.view(np.uint64))
-    assert np.array_equal(dy2.download().view(np.uint64), y.view(np.uint64))
+    assert _same_written(dF2.download(), F, mf)
+    assert _same_written(dy2.download(), y, my)
 
 
 def test_uneven_dif16_body_bit_exact(cfft, oracle):
@@ -83,6 +100,7 @@ def test_uneven_dif16_body_bit_exact(cfft, oracle):
     sched = Schedule(jobs, nin, nf, nout, algo=FftAlgo.Dif16)
     for l in range(9, 15):
         assert sched.class_algo(l) == (FftAlgo.Dif16, min(1 << (l - 1), 1024))
+    assert sched.tile_schedule() == "device_queue"
     dx = DeviceBuffer((nin,), np.float64)
     dF = DeviceBuffer((nf,), np.complex128)
     dy = DeviceBuffer((nout,), np.float64)
@@ -93,11 +111,17 @@ def test_uneven_dif16_body_bit_exact(cfft, oracle):
         synchronize()
         F, y = dF.download(), dy.download()
         _check_against_oracle(oracle, sched, jobs, x, F, y)
-    dF2 = DeviceBuffer((nf,), np.complex128)
-    dy2 = DeviceBuffer((nout,), np.float64)
-    sched.run_plain(dx, dF2, dy2)
-    assert np.array_equal(dF2.download().view(np.uint64), F.view(np.uint64))
-    assert np.array_equal(dy2.download().view(np.uint64), y.view(np.uint64))
+    # every launch form and both tile schedules: the same bits
+    mf, my = _written(jobs, nf, nout)
+    forms = [("device_queue", sched.run_direct), ("device_queue", sched.run_plain),
+             ("static_deal", sched.run), ("static_deal", sched.run_direct)]
+    for kind, run in forms:
+        sched.set_tile_schedule(kind)
+        dF2 = DeviceBuffer((nf,), np.complex128)
+        dy2 = DeviceBuffer((nout,), np.float64)
+        run(dx, dF2, dy2)
+        assert _same_written(dF2.download(), F, mf), kind
+        assert _same_written(dy2.download(), y, my), kind
 
 
 def test_uneven_graph_replays(cfft, oracle):
@@ -119,7 +143,7 @@ def test_uneven_graph_replays(cfft, oracle):
     # new buffer addresses: the graph is captured again
     dy2 = DeviceBuffer((nout,), np.float64)
     sched.run(dx, dF, dy2)
-    assert np.array_equal(dy2.download().view(np.uint64), dy.download().view(np.uint64))
+    assert _same_written(dy2.download(), dy.download(), _written(jobs, nf, nout)[1])
 
 
 def test_uneven_full_workload_roundtrip(cfft):
