@@ -257,57 +257,80 @@ __device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_
   }
 }
 
-// One transform of n = 2^LOG2N points (LOG2N = 8, 9 or 10) on T = n/16
-// lanes of one wavefront (64 / T transforms per wave, each on its own lanes).
-// `tw` is the r16 twiddle block of the 1024-point transform (in LDS): the
-// twiddles of a smaller transform are its entries at column stride 1024 / n
-// (T_n[j] = T_1024[j * 1024 / n], the same sincospi argument).  `x` is the
-// transform's own exchange buffer (R16<LOG2N>::lds_words points).  Same
-// layouts and operation order as r16_transform; no s_barrier.  At n = 256
-// the reference's Dif16 is one core at stride 1 and a radix-16 end at stride
-// 16 (points l + 16 k, in place), so there is one exchange and end_m(s) = s.
-template <int LOG2N, bool FWD>
-__device__ __forceinline__ void r16l_transform(c64 *v, c64 *x, uint32_t l, const c64 *tw) {
+// NV (1 or 2) transforms of n = 2^LOG2N points (LOG2N = 8, 9 or 10) on
+// T = n/16 lanes of one wavefront (64 / T lane groups per wave, each with its
+// own transforms), sharing every twiddle read.  `tw` is the r16 twiddle
+// block of the 1024-point transform (in LDS): the twiddles of a smaller
+// transform are its entries at column stride 1024 / n (T_n[j] =
+// T_1024[j * 1024 / n], the same sincospi argument).  `x` is the group's
+// exchange buffer (R16<LOG2N>::lds_words points; the NV transforms exchange
+// through it in turn).  Same layouts and operation order as r16_transform;
+// no s_barrier.  At n = 256 the reference's Dif16 is one core at stride 1
+// and a radix-16 end at stride 16 (points l + 16 k, in place), so there is
+// one exchange and end_m(s) = s.
+template <int LOG2N, bool FWD, int NV = 1>
+__device__ __forceinline__ void r16l_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                               const c64 *tw) {
   using S = R16<LOG2N>;
   static_assert(S::T <= 64 && LOG2N >= 8, "one transform must fit a wavefront");
   constexpr uint32_t cs = 1u << (10 - LOG2N);  // column stride in the 1024 block
-  butterfly<16, FWD>(v);
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
   sched_fence();
-  twiddle16_x1<FWD>(v, tw, 64, l * cs);
+  if (NV == 2) twiddle16_x2<FWD>(va, vb, tw, 64, l * cs);
+  else twiddle16_x1<FWD>(va, tw, 64, l * cs);
   if constexpr (LOG2N == 8) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      c64 *v = t == 0 ? va : vb;
+      wave_lds_fence();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[r16_pad(16 * l + (uint32_t)k)] = v[k];
+      wave_lds_fence();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + 16 * (uint32_t)k)];
+    }
+    wave_lds_fence();
+    butterfly<16, FWD>(va);
+    if (NV == 2) butterfly<16, FWD>(vb);
+    return;
+  }
+  const uint32_t p = l >> 4, q = l & 15;
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    c64 *v = t == 0 ? va : vb;
     wave_lds_fence();
 #pragma unroll
     for (int k = 0; k < 16; ++k) x[r16_pad(16 * l + (uint32_t)k)] = v[k];
     wave_lds_fence();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + 16 * (uint32_t)k)];
-    wave_lds_fence();
-    butterfly<16, FWD>(v);
-    return;
+    for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
   }
-  const uint32_t p = l >> 4, q = l & 15;
-  wave_lds_fence();
-#pragma unroll
-  for (int k = 0; k < 16; ++k) x[r16_pad(16 * l + (uint32_t)k)] = v[k];
-  wave_lds_fence();
-#pragma unroll
-  for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
-  butterfly<16, FWD>(v);
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
   sched_fence();
-  twiddle16_x1<FWD>(v, tw + 16 * 64, 4, p * cs);
-  wave_lds_fence();
+  if (NV == 2) twiddle16_x2<FWD>(va, vb, tw + 16 * 64, 4, p * cs);
+  else twiddle16_x1<FWD>(va, tw + 16 * 64, 4, p * cs);
 #pragma unroll
-  for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
-  wave_lds_fence();
+  for (int t = 0; t < NV; ++t) {
+    c64 *v = t == 0 ? va : vb;
+    wave_lds_fence();
 #pragma unroll
-  for (uint32_t i = 0; i < S::NB; ++i)
+    for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
+    wave_lds_fence();
 #pragma unroll
-    for (uint32_t k = 0; k < S::REND; ++k)
-      v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
+    for (uint32_t i = 0; i < S::NB; ++i)
+#pragma unroll
+      for (uint32_t k = 0; k < S::REND; ++k)
+        v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
+  }
   wave_lds_fence();
 #pragma unroll
   for (uint32_t i = 0; i < S::NB; ++i) {
-    if constexpr (S::REND > 1) butterfly<S::REND, FWD>(v + i * S::REND);
+    if constexpr (S::REND > 1) {
+      butterfly<S::REND, FWD>(va + i * S::REND);
+      if (NV == 2) butterfly<S::REND, FWD>(vb + i * S::REND);
+    }
   }
 }
 

@@ -59,8 +59,17 @@ constexpr bool r16_class(bool dif16, uint32_t log2n, uint32_t log2_base) {
   return dif16 && log2n >= 8 && log2n <= 13 && log2_base == (log2n < 10 ? log2n : 10);
 }
 
-// groups (jobs) per tile for a class of complex length 2^log2n
+// groups (transform groups of n/16 threads) per tile for a class of
+// complex length 2^log2n, and jobs per tile: one per group, jobs_per_group
+// for the wave-local Dif16 classes (n <= 1024)
 constexpr uint32_t groups_per_tile(uint32_t log2n) { return kThreads / ((1u << log2n) / 16); }
+// jobs a group of the wave-local Dif16 body transforms together: two at
+// n = 1024 (a group is a whole wavefront), one below (two would need more
+// than 256 VGPRs beside the per-lane job pointers of sub-wave groups)
+constexpr uint32_t jobs_per_group(uint32_t log2n) { return log2n >= 10 ? 2u : 1u; }
+constexpr uint32_t jobs_per_tile(uint32_t log2n, bool r16) {
+  return groups_per_tile(log2n) * (r16 && log2n <= 10 ? jobs_per_group(log2n) : 1u);
+}
 
 // sets the kernels' LDS limit on the current device (before any launch)
 hipError_t prepare();

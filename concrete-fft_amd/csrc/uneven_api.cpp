@@ -96,7 +96,7 @@ HostSchedule make_schedule(const std::vector<std::vector<uint32_t>> &jobs_by_cla
     r.log2n = classes[c].log2n;
     r.r16 = classes[c].r16 != 0;
     r.first_tile = (uint32_t)h.tiles.size();
-    const uint32_t G = cfft_ub::groups_per_tile(classes[c].log2n);
+    const uint32_t G = cfft_ub::jobs_per_tile(classes[c].log2n, classes[c].r16 != 0);
     const uint32_t base = (uint32_t)h.order.size();
     h.order.insert(h.order.end(), js.begin(), js.end());
     for (uint32_t i = 0; i < js.size(); i += G) {

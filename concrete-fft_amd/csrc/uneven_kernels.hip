@@ -101,48 +101,82 @@ __device__ __forceinline__ void run_tile(const UClass &C, const UTile &tile,
 
 // ---- Dif16 body ----
 
-// One job of n = 2^LOG2N points (256 <= n <= 1024) on T = n/16 lanes of one
-// wavefront: fold, twist, forward, Fourier store, inverse, untwist, store.
-// `job` is null on lanes whose group has no job; `x` is the group's exchange
-// buffer, `twl` the 1024-point twiddle block in LDS.
-template <int LOG2N>
-__device__ __forceinline__ void job_r16_local(const UClass &C, const UJob *job, const double *in,
-                                              c64 *fourier, double *out, c64 *x, uint32_t l,
-                                              const c64 *twl) {
+// NV (1 or 2) jobs of n = 2^LOG2N points (256 <= n <= 1024) on T = n/16
+// lanes of one wavefront, transformed together (every twiddle, twist and
+// untwist read shared): fold, twist, forward, Fourier store, inverse,
+// untwist, store.
+// A null job is skipped (its lanes compute on zeros); `x` is the group's
+// exchange buffer, `twl` the 1024-point twiddle block in LDS.
+template <int LOG2N, int NV>
+__device__ __forceinline__ void job2_r16_local(const UClass &C, const UJob *ja, const UJob *jb,
+                                               const double *in, c64 *fourier, double *out,
+                                               c64 *x, uint32_t l, const c64 *twl) {
   using S = R16<LOG2N>;
   constexpr uint32_t n = S::n, T = S::T;
-  const bool active = job != nullptr;
-  JobPtrs P{nullptr, nullptr, nullptr};
-  if (active) P = job_ptrs(*job, in, fourier, out);
+  JobPtrs A{nullptr, nullptr, nullptr}, B{nullptr, nullptr, nullptr};
+  if (ja) A = job_ptrs(*ja, in, fourier, out);
+  if (jb) B = job_ptrs(*jb, in, fourier, out);
   const c64 *twist = reinterpret_cast<const c64 *>(C.twist);
   const c64 *untwist = reinterpret_cast<const c64 *>(C.untwist);
-  c64 v[16];
+  c64 va[16], vb[16];
+  // every coefficient load first (their latency overlaps), then the twist
+  // four points at a time (its loads hit the cache)
 #pragma unroll
   for (uint32_t m = 0; m < 16; ++m) {
     const uint32_t j = l + T * m;
-    v[m] = active ? cmul(twist[j], mk(ld_stream(&P.x[j]), ld_stream(&P.x[j + n]))) : mk(0.0, 0.0);
+    va[m] = ja ? mk(ld_stream(&A.x[j]), ld_stream(&A.x[j + n])) : mk(0.0, 0.0);
+    if (NV == 2) vb[m] = jb ? mk(ld_stream(&B.x[j]), ld_stream(&B.x[j + n])) : mk(0.0, 0.0);
   }
-  r16l_transform<LOG2N, true>(v, x, l, twl);
-  if (active) {
+  sched_fence();
 #pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &P.f[l + T * S::end_m(s)]);
-  }
-  c64 u[16];
-  r16_end_to_input<LOG2N>(v, u);
-  r16l_transform<LOG2N, false>(u, x, l, twl);
-  if (active) {
+  for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
 #pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-      const uint32_t j = l + T * S::end_m(s);
-      const c64 r = cmul(untwist[j], u[s]);
-      st_stream(r.re, &P.y[j]);
-      st_stream(r.im, &P.y[j + n]);
+    for (uint32_t m = m0; m < m0 + 4; ++m) {
+      const c64 w = twist[l + T * m];
+      va[m] = cmul(w, va[m]);
+      if (NV == 2) vb[m] = cmul(w, vb[m]);
     }
+    sched_fence();
+  }
+  r16l_transform<LOG2N, true, NV>(va, vb, x, l, twl);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t j = l + T * S::end_m(s);
+    if (ja) st_stream(va[s], &A.f[j]);
+    if (NV == 2 && jb) st_stream(vb[s], &B.f[j]);
+  }
+  sched_fence();  // stores issued before the inverse reuses their registers
+  c64 ua[16], ub[16];
+  r16_end_to_input<LOG2N>(va, ua);
+  if (NV == 2) r16_end_to_input<LOG2N>(vb, ub);
+  r16l_transform<LOG2N, false, NV>(ua, ub, x, l, twl);
+  sched_fence();
+  // untwist eight points at a time (all sixteen loads at once beside both
+  // results exceed the register budget)
+#pragma unroll
+  for (uint32_t s0 = 0; s0 < 16; s0 += 8) {
+#pragma unroll
+    for (uint32_t s = s0; s < s0 + 8; ++s) {
+      const uint32_t j = l + T * S::end_m(s);
+      const c64 w = untwist[j];
+      if (ja) {
+        const c64 r = cmul(w, ua[s]);
+        st_stream(r.re, &A.y[j]);
+        st_stream(r.im, &A.y[j + n]);
+      }
+      if (NV == 2 && jb) {
+        const c64 r = cmul(w, ub[s]);
+        st_stream(r.re, &B.y[j]);
+        st_stream(r.im, &B.y[j + n]);
+      }
+    }
+    sched_fence();
   }
 }
 
-// jobs tile.first .. + tile.count on groups of T lanes; group g uses the
-// exchange buffer at g * lds_words (so wave w's groups share w * 1088)
+// jobs tile.first .. + tile.count, NV per group of T lanes (group g: jobs
+// NV g .. NV g + NV - 1); group g uses the exchange buffer at g * lds_words
+// (so wave w's groups share w * 1088)
 template <int LOG2N>
 __device__ __forceinline__ void group_r16_local(const UClass &C, const UTile &tile,
                                                 const UJob *__restrict__ jobs,
@@ -150,9 +184,16 @@ __device__ __forceinline__ void group_r16_local(const UClass &C, const UTile &ti
                                                 const double *in, c64 *fourier, double *out,
                                                 c64 *xbase, const c64 *twl, uint32_t tid) {
   using S = R16<LOG2N>;
-  const uint32_t g = tid / S::T, l = tid % S::T;
-  const UJob *job = g < tile.count ? &jobs[order[tile.first + g]] : nullptr;
-  job_r16_local<LOG2N>(C, job, in, fourier, out, xbase + g * S::lds_words, l, twl);
+  constexpr int NV = cfft_ub::jobs_per_group(LOG2N);
+  uint32_t g = tid / S::T;
+  const uint32_t l = tid % S::T;
+  // a group that is a whole wavefront: its job pointers are wave-uniform
+  // (scalar registers instead of 64-bit vector pairs)
+  if constexpr (S::T == 64) g = __builtin_amdgcn_readfirstlane(g);
+  const uint32_t j0 = NV * g;
+  const UJob *ja = j0 < tile.count ? &jobs[order[tile.first + j0]] : nullptr;
+  const UJob *jb = NV == 2 && j0 + 1 < tile.count ? &jobs[order[tile.first + j0 + 1]] : nullptr;
+  job2_r16_local<LOG2N, NV>(C, ja, jb, in, fourier, out, xbase + g * S::lds_words, l, twl);
 }
 
 __device__ __forceinline__ void run_group_r16_local(const UClass &C, const UTile &tile,
@@ -223,14 +264,14 @@ __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &til
     c64 v[16];
 #pragma unroll
     for (uint32_t m = 0; m < 16; ++m) v[m] = xc[r16_pad(l + 64 * m)];
-    r16l_transform<10, true>(v, xc, l, twl);
+    r16l_transform<10, true>(v, v, xc, l, twl);
     if (active) {
 #pragma unroll
       for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &P.f[c * 1024 + l + 64 * S::end_m(s)]);
     }
     c64 u[16];
     r16_end_to_input<10>(v, u);
-    r16l_transform<10, false>(u, xc, l, twl);
+    r16l_transform<10, false>(u, u, xc, l, twl);
     wave_lds_fence();
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
@@ -311,7 +352,7 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
   stage_r16_twiddles(tw_block, twl);
   if (queue && threadIdx.x == 0) s_next[0] = part + P * atomicAdd(ctr, 1u);
   __syncthreads();
-  uint32_t t = queue ? s_next[0] : blockIdx.x;
+  uint32_t t = queue ? __builtin_amdgcn_readfirstlane(s_next[0]) : blockIdx.x;
   // queue slots: a slot is written one tile after every thread read it
   for (uint32_t slot = 1; t < ntiles; slot ^= 1u) {
     uint32_t next = 0;
@@ -327,7 +368,7 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
       run_tile_r16_depth(C, tile, jobs, order, in, fourier, out, smem, twl, tid);
     if (queue && threadIdx.x == 0) s_next[slot] = next;
     __syncthreads();
-    t = queue ? s_next[slot] : t + gridDim.x;
+    t = queue ? __builtin_amdgcn_readfirstlane(s_next[slot]) : t + gridDim.x;
   }
 }
 
