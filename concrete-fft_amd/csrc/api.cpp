@@ -57,6 +57,7 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_pos) (void)hipFree(d_pos);
   if (d_twist) (void)hipFree(d_twist);
   if (d_tw_r16) (void)hipFree(d_tw_r16);
+  if (d_tw_base1024) (void)hipFree(d_tw_base1024);
   if (d_tw_r8) (void)hipFree(d_tw_r8);
   if (d_untwist) (void)hipFree(d_untwist);
   if (d_tw128) (void)hipFree(d_tw128);
@@ -202,6 +203,17 @@ int build_f64_plan(cfft_plan_s *p) {
     }
     CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
     CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
+  }
+  if (p->algo == CFFT_DIF16 && p->base_n == 1024 && n > 1024 && n <= 8192) {
+    const size_t mul = n / 1024;
+    std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);
+    for (size_t k = 0; k < 16; ++k) {
+      for (size_t l = 0; l < 64; ++l) blk[k * 64 + l] = tw[k * l * mul];
+      for (size_t q = 0; q < 4; ++q) blk[1024 + k * 4 + q] = tw[16 * k * q * mul];
+    }
+    CFFT_HIP(hipMalloc(&p->d_tw_base1024, blk.size() * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(p->d_tw_base1024, blk.data(), blk.size() * sizeof(dc64),
+                       hipMemcpyHostToDevice));
   }
   if (p->algo == CFFT_DIF8 && p->base_n == n && cfft_launch::r8_supported(log2n)) {
     std::vector<cfft_host::C64> blk(8 * 128 + 8 * 16 + 8 * 2);
