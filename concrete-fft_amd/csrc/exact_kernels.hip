@@ -246,156 +246,141 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
   }
 }
 
-// ---- depth form for Dif16 plans with n = 2048, base_n = 1024 (N = 4096) ----
-// The plan is one radix-2 depth pass (src/unordered.rs:391-489) around two
-// 1024-point Dif16 base transforms.  A transform of 2048 points runs on 128
-// threads: thread t computes the depth butterflies j = t + 128 i (i < 8) of
-// the points j and 1024 + j in registers, writes the two chunks to LDS
-// (chunk k at k * 1088 points), one workgroup barrier, then wave c runs chunk
-// c's base transform in registers (wave-local Dif16).  The inverse mirrors
-// it.  Same operation order as the pass-program executor, so the same bits.
-constexpr uint32_t kDN = 2048;
-constexpr uint32_t kDCH = R16<10>::lds_words;  // LDS points per 1024-point chunk
-constexpr size_t kDLds = (kXTw + 4 * 2 * kDCH) * sizeof(c64);
+// ---- wave-local form for Dif16 plans with n = 2048, base_n = 1024 (N = 4096) ----
+// One 2048-point transform per wavefront (r16l_2048_transform: the plan's
+// radix-2 depth pass in registers around two 1024-point base transforms run
+// together).  Forward: one wave per (product, limb transform), 8 waves per
+// workgroup.  Inverse: one wave per (product, weight k); the rounded,
+// shifted parts of the four weights meet in LDS, chunk 0's positions first,
+// then chunk 1's, and each of the product's waves adds and stores a quarter
+// of them.  (Both kernels keep a few dozen VGPRs in scratch: the paired
+// 1024-point transforms alone take 240 of the 256 available.)
+constexpr uint32_t kWN = 2048;
 
-// Forward: 4 transforms (t = 8 p + l: l < 4 limb l of a, else limb l - 4 of
-// b) per 512-thread workgroup, 2 waves each.
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_exact_fwd_d2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
+__global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
                c64 *__restrict__ F, double *margin, uint64_t batch, const c64 *__restrict__ tw,
                const c64 *__restrict__ tw_blk, const c64 *__restrict__ twist) {
   using S = R16<10>;
-  constexpr uint32_t n = kDN;
+  constexpr uint32_t n = kWN;
   extern __shared__ c64 smem[];
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_blk[i];
-  const uint32_t g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);  // transform group
-  const uint32_t t = threadIdx.x & 127u;
-  const uint64_t u = (uint64_t)blockIdx.x * 4 + g;
-  const bool active = u < batch * 8;
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t u = (uint64_t)blockIdx.x * kXW + wave;  // 8 p + limb transform
+  if (u >= batch * 8) return;
+  c64 *x = smem + kXTw + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
   const uint64_t p = u >> 3;
   const uint32_t li = (uint32_t)(u & 7), lb = li & 3;
-  if (active && li == 0 && t == 0 && margin) margin[p] = 0.0;
-  const uint64_t *src = (li < 4 ? A : B) + (active ? p : 0) * (2ull * n);
-  c64 *X = smem + kXTw + g * 2 * kDCH;
-  {
-    c64 v[8][2];
+  if (li == 0 && l == 0 && margin) margin[p] = 0.0;
+  const uint64_t *src = (li < 4 ? A : B) + p * (2ull * n);
+  c64 va[16], vb[16];
+  // each half (points j and 1024 + j): its coefficient loads first, then
+  // limbs and twist four points at a time
 #pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) {
-      const uint32_t j = t + 128 * i;
+  for (uint32_t h = 0; h < 2; ++h) {
+    c64 *v = h == 0 ? va : vb;
+    uint64_t r[32];
 #pragma unroll
-      for (uint32_t k = 0; k < 2; ++k) {
-        const uint32_t q = k * 1024 + j;
-        v[i][k] = active ? cmul(twist[q], mk((double)(int32_t)limb16(src[q], lb),
-                                             (double)(int32_t)limb16(src[q + n], lb)))
-                         : mk(0.0, 0.0);
-      }
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = h * 1024 + l + 64 * m;
+      r[2 * m] = src[j];
+      r[2 * m + 1] = src[j + n];
     }
+    sched_fence();
 #pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) {
-      const uint32_t j = t + 128 * i;
-      butterfly<2, true>(v[i]);
-      v[i][1] = cmul(tw[j], v[i][1]);
-      X[r16_pad(j)] = v[i][0];  // chunk bitrev(0) = 0
-      X[kDCH + r16_pad(j)] = v[i][1];
+    for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
+#pragma unroll
+      for (uint32_t m = m0; m < m0 + 4; ++m)
+        v[m] = cmul(twist[h * 1024 + l + 64 * m],
+                    mk((double)(int32_t)limb16(r[2 * m], lb),
+                       (double)(int32_t)limb16(r[2 * m + 1], lb)));
+      sched_fence();
     }
   }
-  __syncthreads();  // also orders the twiddle block staging
-  const uint32_t c = __builtin_amdgcn_readfirstlane(t >> 6), l = t & 63u;
-  c64 *xc = X + c * kDCH;
-  c64 v[16];
+  r16l_2048_transform<true>(va, vb, x, l, ltw, tw);
+  c64 *out = F + u * n;
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) v[m] = xc[r16_pad(l + 64 * m)];
-  r16l_transform<10, true>(v, v, xc, l, ltw);
-  if (active) {
-    c64 *out = F + u * n + c * 1024;
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) out[l + 64 * S::end_m(s)] = v[s];
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t q = l + 64 * S::end_m(s);
+    out[q] = va[s];
+    out[1024 + q] = vb[s];
   }
 }
 
-// Inverse: one workgroup per product, wave w = 2 k + c: weight k's chunk c.
-// Each weight's two waves sum their Fourier terms, run the chunk inverses,
-// exchange through LDS for the depth pass, untwist, round and shift by 16 k;
-// the weights' parts meet in LDS and waves 0-1 add and store them.
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_exact_inv_d2(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *margin,
+__global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_exact_inv_w2(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *margin,
                uint64_t batch, const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk,
                const c64 *__restrict__ untwist) {
   using S = R16<10>;
-  constexpr uint32_t n = kDN;
+  constexpr uint32_t n = kWN;
   extern __shared__ c64 smem[];
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_blk[i];
   __syncthreads();
-  const uint64_t p = blockIdx.x;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint32_t k = wave >> 1, c = wave & 1, l = threadIdx.x & 63u;
-  const uint32_t t = threadIdx.x & 127u;  // thread within the weight's pair of waves
-  c64 *X = smem + kXTw + k * 2 * kDCH;
-  c64 *xc = X + c * kDCH;
-  const c64 *fp = F + p * 8ull * n + c * 1024;
-  {
-    c64 v[16];
+  const uint64_t p = (uint64_t)blockIdx.x * (kXW / 4) + (wave >> 2);
+  const uint32_t k = wave & 3;
+  const bool active = p < batch;  // every wave reaches the barriers below
+  c64 *x = smem + kXTw + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  const c64 *fp = F + (active ? p : 0) * 8ull * n;
+  c64 va[16], vb[16];
 #pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t q = l + 64 * m;
-      c64 d = cmul(fp[q], fp[(4 + k) * n + q]);
-      for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(fp[i * n + q], fp[(4 + k - i) * n + q]));
-      v[m] = d;
-      if ((m & 3) == 3) sched_fence();  // bounds the loads in flight
+  for (uint32_t m = 0; m < 16; ++m) {
+    const uint32_t q = l + 64 * m;
+    c64 da = cmul(fp[q], fp[(4 + k) * n + q]);
+    c64 db = cmul(fp[1024 + q], fp[(4 + k) * n + 1024 + q]);
+    for (uint32_t i = 1; i <= k; ++i) {
+      da = cadd(da, cmul(fp[i * n + q], fp[(4 + k - i) * n + q]));
+      db = cadd(db, cmul(fp[i * n + 1024 + q], fp[(4 + k - i) * n + 1024 + q]));
     }
-    r16l_transform<10, false>(v, v, xc, l, ltw);
-    wave_lds_fence();
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = v[s];
+    va[m] = da;
+    vb[m] = db;
+    sched_fence();  // bounds the loads in flight
   }
-  __syncthreads();
+  r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
+  // the transform's last LDS reads are behind its final wave fence; the
+  // wave's buffer now carries 32 words per lane, one half at a time, each
+  // word written as soon as it is rounded
+  uint64_t *xb = reinterpret_cast<uint64_t *>(x);
+  uint64_t *cp = C + (active ? p : 0) * (2ull * n);
   double worst = 0.0;
-  uint64_t part[32];
 #pragma unroll
-  for (uint32_t i = 0; i < 8; ++i) {
-    const uint32_t j = t + 128 * i;
-    c64 v[2] = {X[r16_pad(j)], X[kDCH + r16_pad(j)]};
-    c64 w = tw[j];
-    w.im = -w.im;
-    v[1] = cmul(w, v[1]);
-    butterfly<2, false>(v);
+  for (uint32_t h = 0; h < 2; ++h) {
+    const c64 *v = h == 0 ? va : vb;
 #pragma unroll
-    for (uint32_t kk = 0; kk < 2; ++kk) {
-      const uint32_t q = kk * 1024 + j;
-      const c64 r = cmul(untwist[q], v[kk]);
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t q = h * 1024 + l + 64 * S::end_m(s);
+      const c64 r = cmul(untwist[q], v[s]);
       const double x0 = round(r.re), x1 = round(r.im);
       worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
-      part[4 * i + 2 * kk] = (uint64_t)(int64_t)x0 << (16 * k);
-      part[4 * i + 2 * kk + 1] = (uint64_t)(int64_t)x1 << (16 * k);
+      xb[(2 * s) * 64 + l] = (uint64_t)(int64_t)x0 << (16 * k);
+      xb[(2 * s + 1) * 64 + l] = (uint64_t)(int64_t)x1 << (16 * k);
+      if ((s & 3) == 3) sched_fence();
     }
-  }
-  __syncthreads();  // every weight's depth-pass reads of its region are done
-  uint64_t *xb = reinterpret_cast<uint64_t *>(X);  // 32 x 128 words fit in 2 chunks
+    __syncthreads();
+    if (active) {
+      // wave k adds and stores slots 4k .. 4k + 3 of the four weights' words
+      const uint64_t *b0 = reinterpret_cast<const uint64_t *>(x - k * S::lds_words);
+      const uint32_t stride = S::lds_words * 2;  // one buffer, in words
 #pragma unroll
-  for (uint32_t q = 0; q < 32; ++q) xb[q * 128 + t] = part[q];
-  __syncthreads();
-  if (k == 0) {
-    const uint64_t *b1 = reinterpret_cast<const uint64_t *>(smem + kXTw + 1 * 2 * kDCH);
-    const uint64_t *b2 = reinterpret_cast<const uint64_t *>(smem + kXTw + 2 * 2 * kDCH);
-    const uint64_t *b3 = reinterpret_cast<const uint64_t *>(smem + kXTw + 3 * 2 * kDCH);
-    uint64_t *cp = C + p * (2ull * n);
-#pragma unroll
-    for (uint32_t i = 0; i < 8; ++i) {
-      const uint32_t j = t + 128 * i;
-#pragma unroll
-      for (uint32_t kk = 0; kk < 2; ++kk) {
-        const uint32_t q = kk * 1024 + j;
-        const uint32_t w0 = (4 * i + 2 * kk) * 128 + t, w1 = w0 + 128;
-        st_stream(part[4 * i + 2 * kk] + b1[w0] + b2[w0] + b3[w0], &cp[q]);
-        st_stream(part[4 * i + 2 * kk + 1] + b1[w1] + b2[w1] + b3[w1], &cp[q + n]);
+      for (uint32_t e = 0; e < 4; ++e) {
+        const uint32_t s = 4 * k + e;
+        const uint32_t q = h * 1024 + l + 64 * S::end_m(s);
+        const uint32_t w0 = (2 * s) * 64 + l, w1 = (2 * s + 1) * 64 + l;
+        st_stream(b0[w0] + b0[stride + w0] + b0[2 * stride + w0] + b0[3 * stride + w0], &cp[q]);
+        st_stream(b0[w1] + b0[stride + w1] + b0[2 * stride + w1] + b0[3 * stride + w1],
+                  &cp[q + n]);
       }
     }
+    __syncthreads();  // the buffers are rewritten for the second half
   }
   if (margin) {
     for (uint32_t off = 1; off < 64; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
-    if (l == 0)
+    if (active && l == 0)
       atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
                 (unsigned long long)__double_as_longlong(worst));
   }
@@ -489,26 +474,29 @@ hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, 
   return hipGetLastError();
 }
 
-// Depth form: Dif16 plan with n = 2048 and base_n = 1024; tw is the plan's
-// full table, tw_blk its 1024-point base block.
-hipError_t exact_polymul_d2(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
+// Wave-local form: Dif16 plan with n = 2048 and base_n = 1024; tw is the
+// plan's full table, tw_blk its 1024-point base block.
+hipError_t exact_polymul_w2(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
                             uint64_t batch, const dc64 *tw, const dc64 *tw_blk,
                             const dc64 *twist, const dc64 *untwist, dc64 *F, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_fwd_d2), kDLds);
+  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_fwd_w2), kXLds);
   if (e == hipSuccess)
-    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_inv_d2), kDLds);
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_inv_w2), kXLds);
   if (e != hipSuccess) return e;
   const c64 *t = reinterpret_cast<const c64 *>(tw);
   const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
   c64 *f = reinterpret_cast<c64 *>(F);
   const uint64_t transforms = batch * 8;
-  hipLaunchKernelGGL(k_exact_fwd_d2, dim3((uint32_t)((transforms + 3) / 4)), dim3(512), kDLds, s,
-                     a, b, f, margin, batch, t, tb, reinterpret_cast<const c64 *>(twist));
+  hipLaunchKernelGGL(k_exact_fwd_w2, dim3((uint32_t)((transforms + kXW - 1) / kXW)),
+                     dim3(64 * kXW), kXLds, s, a, b, f, margin, batch, t, tb,
+                     reinterpret_cast<const c64 *>(twist));
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_exact_inv_d2, dim3((uint32_t)batch), dim3(512), kDLds, s, f, c, margin,
-                     batch, t, tb, reinterpret_cast<const c64 *>(untwist));
+  const uint64_t per_wg = kXW / 4;  // products per workgroup (4 waves each)
+  hipLaunchKernelGGL(k_exact_inv_w2, dim3((uint32_t)((batch + per_wg - 1) / per_wg)),
+                     dim3(64 * kXW), kXLds, s, f, c, margin, batch, t, tb,
+                     reinterpret_cast<const c64 *>(untwist));
   return hipGetLastError();
 }
 

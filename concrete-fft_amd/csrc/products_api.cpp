@@ -15,7 +15,7 @@ hipError_t exact_polymul(const cfft_prog::Program &fwd, const cfft_prog::Program
                          const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
                          uint64_t batch, const dc64 *tw, const dc64 *twist, const dc64 *untwist,
                          dc64 *F, hipStream_t s);
-hipError_t exact_polymul_d2(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
+hipError_t exact_polymul_w2(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
                             uint64_t batch, const dc64 *tw, const dc64 *tw_blk,
                             const dc64 *twist, const dc64 *untwist, dc64 *F, hipStream_t s);
 hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
@@ -75,7 +75,7 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
     e = cfft_launch::exact_polymul_r16(a, b, c, margin, batch, plan->d_tw_r16, plan->d_twist,
                                        reinterpret_cast<cfft_launch::dc64 *>(F), s);
   else if (plan->n == 2048 && plan->d_tw_base1024)  // Dif16 plan, base_n == 1024
-    e = cfft_launch::exact_polymul_d2(a, b, c, margin, batch, plan->d_tw, plan->d_tw_base1024,
+    e = cfft_launch::exact_polymul_w2(a, b, c, margin, batch, plan->d_tw, plan->d_tw_base1024,
                                       plan->d_twist, plan->d_untwist,
                                       reinterpret_cast<cfft_launch::dc64 *>(F), s);
   else

@@ -334,4 +334,62 @@ __device__ __forceinline__ void r16l_transform(c64 *va, c64 *vb, c64 *x, uint32_
   }
 }
 
+// A Dif16 plan with n = 2048 and base_n = 1024 on one wavefront.  The
+// plan is one radix-2 depth pass (src/unordered.rs:391-489) around two
+// 1024-point base transforms; the depth butterfly pairs the points j and
+// 1024 + j, which lane l holds in the same slot (va[m] = point l + 64 m,
+// vb[m] = point 1024 + l + 64 m), so the depth pass runs in registers and
+// the two base transforms run together (NV = 2, twiddle reads shared).
+//   forward: in  va[m], vb[m] as above; out va[s] = point l + 64 end_m(s),
+//            vb[s] = point 1024 + l + 64 end_m(s)   (plan order)
+//   inverse: in  va[m] = point l + 64 m, vb[m] = point 1024 + l + 64 m (plan
+//            order); out va[s] = point q, vb[s] = point 1024 + q with
+//            q = l + 64 end_m(s)
+// `tw` is the plan's full table T[j] (the depth twiddles T[j], j < 1024, in
+// global memory), `twl` the 1024-point r16 block in LDS, `x` the wave's
+// exchange buffer (R16<10>::lds_words points).
+template <bool FWD>
+__device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                                    const c64 *twl, const c64 *__restrict__ tw) {
+  using S = R16<10>;
+  if constexpr (FWD) {
+    // DEPTH_FWD radix 2: (a + b, (a - b) T[j]) to chunks 0 and 1
+#pragma unroll
+    for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
+      c64 w[4];
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) w[e] = tw[l + 64 * (m0 + e)];
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        c64 t2[2] = {va[m0 + e], vb[m0 + e]};
+        butterfly<2, true>(t2);
+        va[m0 + e] = t2[0];
+        vb[m0 + e] = cmul(w[e], t2[1]);
+      }
+      sched_fence();
+    }
+    r16l_transform<10, true, 2>(va, vb, x, l, twl);
+  } else {
+    r16l_transform<10, false, 2>(va, vb, x, l, twl);
+    // DEPTH_INV radix 2: conj(T[q]) on chunk 1's point before the butterfly
+#pragma unroll
+    for (uint32_t s0 = 0; s0 < 16; s0 += 4) {
+      c64 w[4];
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        w[e] = tw[l + 64 * S::end_m(s0 + e)];
+        w[e].im = -w[e].im;
+      }
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        c64 t2[2] = {va[s0 + e], cmul(w[e], vb[s0 + e])};
+        butterfly<2, false>(t2);
+        va[s0 + e] = t2[0];
+        vb[s0 + e] = t2[1];
+      }
+      sched_fence();
+    }
+  }
+}
+
 }  // namespace cfft_dev
