@@ -352,6 +352,9 @@ template <bool FWD>
 __device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
                                                     const c64 *twl, const c64 *__restrict__ tw) {
   using S = R16<10>;
+  // a forward and an inverse of one job read the same depth twiddles; merged
+  // loads would stay live across a whole transform
+  tw = launder(tw);
   if constexpr (FWD) {
     // DEPTH_FWD radix 2: (a + b, (a - b) T[j]) to chunks 0 and 1
 #pragma unroll

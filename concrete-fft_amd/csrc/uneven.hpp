@@ -68,7 +68,9 @@ constexpr uint32_t groups_per_tile(uint32_t log2n) { return kThreads / ((1u << l
 // than 256 VGPRs beside the per-lane job pointers of sub-wave groups)
 constexpr uint32_t jobs_per_group(uint32_t log2n) { return log2n >= 10 ? 2u : 1u; }
 constexpr uint32_t jobs_per_tile(uint32_t log2n, bool r16) {
-  return groups_per_tile(log2n) * (r16 && log2n <= 10 ? jobs_per_group(log2n) : 1u);
+  // a Dif16 class at n = 2048 runs one job per wavefront (8 per tile)
+  return r16 && log2n == 11 ? kThreads / 64
+                            : groups_per_tile(log2n) * (r16 && log2n <= 10 ? jobs_per_group(log2n) : 1u);
 }
 
 // sets the kernels' LDS limit on the current device (before any launch)

@@ -196,6 +196,71 @@ __device__ __forceinline__ void group_r16_local(const UClass &C, const UTile &ti
   job2_r16_local<LOG2N, NV>(C, ja, jb, in, fourier, out, xbase + g * S::lds_words, l, twl);
 }
 
+// One job of n = 2048 points on one wavefront (r16l_2048_transform: the
+// plan's radix-2 depth pass in registers around its two 1024-point base
+// transforms, run together).  Lane l holds the points l + 64 m and
+// 1024 + l + 64 m.  A null job is skipped (its lanes compute on zeros).
+__device__ __forceinline__ void job_r16_2048(const UClass &C, const UJob *job, const double *in,
+                                             c64 *fourier, double *out, c64 *x, uint32_t l,
+                                             const c64 *twl) {
+  using S = R16<10>;
+  constexpr uint32_t n = 2048;
+  JobPtrs P{nullptr, nullptr, nullptr};
+  if (job) P = job_ptrs(*job, in, fourier, out);
+  const c64 *tw = reinterpret_cast<const c64 *>(C.tw);
+  const c64 *twist = reinterpret_cast<const c64 *>(C.twist);
+  const c64 *untwist = reinterpret_cast<const c64 *>(C.untwist);
+  c64 va[16], vb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    const uint32_t j = l + 64 * m;
+    va[m] = job ? mk(ld_stream(&P.x[j]), ld_stream(&P.x[j + n])) : mk(0.0, 0.0);
+    vb[m] = job ? mk(ld_stream(&P.x[1024 + j]), ld_stream(&P.x[1024 + j + n])) : mk(0.0, 0.0);
+  }
+  sched_fence();
+#pragma unroll
+  for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
+#pragma unroll
+    for (uint32_t m = m0; m < m0 + 4; ++m) {
+      const uint32_t j = l + 64 * m;
+      va[m] = cmul(twist[j], va[m]);
+      vb[m] = cmul(twist[1024 + j], vb[m]);
+    }
+    sched_fence();
+  }
+  r16l_2048_transform<true>(va, vb, x, l, twl, tw);
+  if (job) {
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t q = l + 64 * S::end_m(s);
+      st_stream(va[s], &P.f[q]);
+      st_stream(vb[s], &P.f[1024 + q]);
+    }
+  }
+  sched_fence();  // stores issued before the inverse reuses their registers
+  c64 ua[16], ub[16];
+  r16_end_to_input<10>(va, ua);
+  r16_end_to_input<10>(vb, ub);
+  r16l_2048_transform<false>(ua, ub, x, l, twl, tw);
+  sched_fence();
+#pragma unroll
+  for (uint32_t s0 = 0; s0 < 16; s0 += 4) {
+#pragma unroll
+    for (uint32_t s = s0; s < s0 + 4; ++s) {
+      const uint32_t q = l + 64 * S::end_m(s);
+      const c64 ra = cmul(untwist[q], ua[s]);
+      const c64 rb = cmul(untwist[1024 + q], ub[s]);
+      if (job) {
+        st_stream(ra.re, &P.y[q]);
+        st_stream(ra.im, &P.y[q + n]);
+        st_stream(rb.re, &P.y[1024 + q]);
+        st_stream(rb.im, &P.y[1024 + q + n]);
+      }
+    }
+    sched_fence();
+  }
+}
+
 __device__ __forceinline__ void run_group_r16_local(const UClass &C, const UTile &tile,
                                                     const UJob *__restrict__ jobs,
                                                     const uint32_t *__restrict__ order,
@@ -204,7 +269,13 @@ __device__ __forceinline__ void run_group_r16_local(const UClass &C, const UTile
   switch (C.log2n) {
     case 8: group_r16_local<8>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
     case 9: group_r16_local<9>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
-    default: group_r16_local<10>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
+    case 10: group_r16_local<10>(C, tile, jobs, order, in, fourier, out, xbase, twl, tid); break;
+    default: {
+      // n = 2048: one job per wave, the wave's exchange buffer at 1088 w
+      const uint32_t g = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const UJob *job = g < tile.count ? &jobs[order[tile.first + g]] : nullptr;
+      job_r16_2048(C, job, in, fourier, out, xbase + g * R16<10>::lds_words, tid & 63u, twl);
+    } break;
   }
 }
 
@@ -314,7 +385,6 @@ __device__ __forceinline__ void run_tile_r16_depth(const UClass &C, const UTile 
                                                    const double *in, c64 *fourier, double *out,
                                                    c64 *smem, const c64 *twl, uint32_t tid) {
   switch (C.log2n) {
-    case 11: tile_r16_depth<11>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     case 12: tile_r16_depth<12>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
     default: tile_r16_depth<13>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
   }
@@ -362,7 +432,7 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
     // opaque per tile: address arithmetic of the bodies is not hoisted out
     // of the loop (it would stay live across it and spill)
     const uint32_t tid = launder_u32(threadIdx.x);
-    if (C.log2n <= 10)
+    if (C.log2n <= 11)
       run_group_r16_local(C, tile, jobs, order, in, fourier, out, smem, twl, tid);
     else
       run_tile_r16_depth(C, tile, jobs, order, in, fourier, out, smem, twl, tid);
@@ -383,7 +453,7 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_tiles(
   const UClass &C = classes[tile.cls];
   stage_r16_twiddles(tw_block, twl);
   __syncthreads();
-  if (C.log2n <= 10)
+  if (C.log2n <= 11)
     run_group_r16_local(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
   else
     run_tile_r16_depth(C, tile, jobs, order, in, fourier, out, smem, twl, threadIdx.x);
