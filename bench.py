@@ -31,7 +31,7 @@ transforms of 32-bit limbs, with both results checked equal and a sample
 checked against the u64 schoolbook.  `ordered` times the ordered c64 plan at
 n = 1024, batch 1 (fwd and inv per call) and reports its error against the
 oracle and its round trip.  `transforms` reports batched unordered forward
-transforms (n = 1024, batch 8192) per second for several plans.  `--mode copy`
+transforms (n = 1024 x 8192 and n = 2048 x 4096) per second for several plans.  `--mode copy`
 runs one
 device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
 byte counters).
@@ -273,26 +273,29 @@ def bench_exact(steps, warmup):
 
 
 def bench_transforms(steps=50):
-    """Batched unordered forward transforms, n = 1024 (N = 2048), batch 8192,
-    in place on device: transforms/s and HBM fraction at 32 n bytes each."""
+    """Batched unordered forward transforms in place on device, 64 Mi points
+    per call: n = 1024 (N = 2048) x 8192 and n = 2048 (N = 4096) x 4096;
+    transforms/s and HBM fraction at 32 n bytes each."""
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
-    n, batch = 1024, 8192
     rng = np.random.default_rng(6)
     s = Stream()
-    x = DeviceBuffer.from_host(rng.uniform(-1, 1, (batch, 2 * n)).view(np.complex128), s)
+    x = DeviceBuffer.from_host(rng.uniform(-1, 1, (8192, 2048)).view(np.complex128), s)
     out = {}
-    for algo, base in ((FftAlgo.Dif16, 1024), (FftAlgo.Dif8, 1024), (FftAlgo.Dit16, 1024),
-                       (FftAlgo.Dif16, 512)):
+    for n, algo, base in ((1024, FftAlgo.Dif16, 1024), (1024, FftAlgo.Dif8, 1024),
+                          (1024, FftAlgo.Dit16, 1024), (1024, FftAlgo.Dif16, 512),
+                          (2048, FftAlgo.Dif16, 1024), (2048, FftAlgo.Dif8, 1024)):
+        batch = 8192 * 1024 // n
         plan = Plan(n, Method.UserProvided(algo, base))
         f = lambda: check(lib.cfft_fwd(plan.handle, x.dptr, batch, n, s.handle))
         ms = float(np.median(time_runs(f, s, steps, 3)))
         gbs = 32 * n * batch / (ms * 1e-3) / 1e9
-        out["%s/%d" % (algo.name, base)] = {"transforms_per_s": round(batch / (ms * 1e-3), 1),
-                                           "ms": round(ms, 4),
-                                           "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
-    return {"workload": "unordered fwd, n = 1024, batch 8192, in place on device", "plans": out}
+        out["n=%d %s/%d" % (n, algo.name, base)] = {
+            "transforms_per_s": round(batch / (ms * 1e-3), 1), "ms": round(ms, 4),
+            "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return {"workload": "unordered fwd in place on device, 8 Mi complex points per call",
+            "plans": out}
 
 
 def bench_ordered(steps=200):

@@ -267,6 +267,10 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   if (p->d_tw_r8)
     return hip_error(cfft_launch::r8_transform(fwd, d, batch, stride, p->d_tw_r8, s),
                      "r8_transform");
+  if (p->n == 2048 && p->d_tw_base1024)  // Dif16, base_n = 1024
+    return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
+                                                  p->d_tw_base1024, s),
+                     "w2048_transform");
   if (fwd) {
     for (const Pass &P : p->fwd_global)
       CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));

@@ -286,9 +286,64 @@ k_r8_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   }
 }
 
+// ---- Dif16 plan with n = 2048, base_n = 1024: one transform per wavefront ----
+// r16l_2048_transform (the plan's radix-2 depth pass in registers around its
+// two 1024-point base transforms, run together), 8 waves per workgroup
+// sharing the 1024-point twiddle block staged in LDS.  In place: each wave
+// reads its whole transform before writing it.
+template <bool FWD>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_w2048_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
+                  const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk) {
+  using S = R16<10>;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t t = (uint64_t)blockIdx.x * kPW + wave;
+  if (t >= batch) return;
+  c64 *x = smem + kTwWords + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  c64 *d = data + t * stride;
+  c64 va[16], vb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    va[m] = ld_stream(&d[l + 64 * m]);
+    vb[m] = ld_stream(&d[1024 + l + 64 * m]);
+  }
+  r16l_2048_transform<FWD>(va, vb, x, l, ltw, tw);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t q = l + 64 * S::end_m(s);
+    st_stream(va[s], &d[q]);
+    st_stream(vb[s], &d[1024 + q]);
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
+
+hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
+                           const dc64 *tw_blk, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const void *k = fwd ? reinterpret_cast<const void *>(k_w2048_transform<true>)
+                      : reinterpret_cast<const void *>(k_w2048_transform<false>);
+  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  if (e != hipSuccess) return e;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
+  const dim3 grid((uint32_t)((batch + kPW - 1) / kPW));
+  if (fwd)
+    hipLaunchKernelGGL(k_w2048_transform<true>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+                       stride, t, tb);
+  else
+    hipLaunchKernelGGL(k_w2048_transform<false>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+                       stride, t, tb);
+  return hipGetLastError();
+}
 
 bool r16_supported(uint32_t log2n) { return log2n >= 9 && log2n <= 12; }
 bool r8_supported(uint32_t log2n) { return log2n == 10; }

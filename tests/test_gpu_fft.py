@@ -85,10 +85,10 @@ def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
         assert np.max(np.abs(w / n - x)) < 1e-12
 
 
-def test_device_path_with_stride(cfft, oracle):
+@pytest.mark.parametrize("n,stride,batch", [(1024, 1100, 7), (2048, 2050, 11)])
+def test_device_path_with_stride(cfft, oracle, n, stride, batch):
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
-    n, stride, batch = 1024, 1100, 7
     rng = np.random.default_rng(1)
     x = rand_c(rng, (batch, stride))
     plan = Plan(n, Method.UserProvided(FftAlgo.Dif16, 1024))
@@ -104,6 +104,11 @@ def test_device_path_with_stride(cfft, oracle):
     ref = oracle.unordered_fwd(np.ascontiguousarray(x[:, :n]), 6, 1024)
     assert np.array_equal(y[:, :n], ref)
     assert np.array_equal(y[:, n:], x[:, n:]), "padding between transforms was touched"
+    cfft_amd.check(cfft_amd.lib.cfft_inv(plan.handle, d2.dptr, batch, stride, s.handle))
+    s.synchronize()
+    w = d2.download()
+    assert np.array_equal(w[:, :n], oracle.unordered_inv(ref, 6, 1024))
+    assert np.array_equal(w[:, n:], x[:, n:])
 
 
 @pytest.mark.parametrize("n", [256, 512, 1024])
