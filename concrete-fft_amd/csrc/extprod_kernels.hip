@@ -15,6 +15,7 @@
 
 
 #include "launch.hpp"
+#include "r16_exec.hpp"
 #include "stockham_exec.hpp"
 
 using namespace cfft_dev;
@@ -199,6 +200,200 @@ __global__ void __launch_bounds__(512) k_extprod_cols(const c64 *__restrict__ ke
   run_program<E>(inv, lds, lt, T, tw, load, store);
 }
 
+// ---- split form on Dif16 plans with base_n = 1024, n = 1024 or 2048 ----
+// One wavefront per row transform and per column: the wave-local
+// register-resident Dif16 transform (n = 1024) or r16l_2048_transform
+// (n = 2048: the plan's radix-2 depth pass in registers around its two
+// 1024-point chunks), W = blockDim.x / 64 waves per workgroup sharing the
+// 1024-point twiddle block in LDS.  The split form serves small batches:
+// up to 8 rows share one workgroup, more spread one wave per workgroup and
+// W grows only once every CU has a row (latency first).
+// Same loads, accumulation order and stores as the generic split kernels
+// above, so the same bits.
+constexpr uint32_t kETw = 16 * 64 + 16 * 4;
+constexpr size_t split_w_lds(uint32_t waves) {
+  return (kETw + waves * R16<10>::lds_words) * sizeof(c64);
+}
+
+// copies the twiddle block into LDS with every load of a lane issued before
+// its first LDS store (a one-wave workgroup would otherwise wait for 17
+// dependent load round trips); callers barrier after
+__device__ __forceinline__ void stage_block(const c64 *__restrict__ src, c64 *dst) {
+  constexpr uint32_t kPer = (kETw + 63) / 64;
+  c64 t[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = threadIdx.x + k * blockDim.x;
+    t[k] = src[i < kETw ? i : kETw - 1];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = threadIdx.x + k * blockDim.x;
+    if (i < kETw) dst[i] = t[k];
+  }
+}
+
+template <int LOG2N>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_extprod_rows_w(const uint64_t *__restrict__ glwe, c64 *__restrict__ S, uint64_t nrows,
+                 uint32_t levels, uint32_t base_log, const c64 *__restrict__ tw,
+                 const c64 *__restrict__ tw_blk, const c64 *__restrict__ twist) {
+  using R = R16<10>;
+  constexpr uint32_t n = 1u << LOG2N;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  stage_block(tw_blk, ltw);
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;  // (p k1 + poly) levels + lvl
+  if (t >= nrows) return;
+  c64 *x = smem + kETw + wave * R::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  const uint32_t lvl = (uint32_t)(t % levels);
+  const uint64_t *src = glwe + (t / levels) * 2ull * n;
+  c64 *dst = S + t * n;
+  auto digit_pt = [&](uint32_t j) -> c64 {
+    return cmul(twist[j],
+                mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
+                   (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
+  };
+  if constexpr (LOG2N == 10) {
+    c64 v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) v[m] = digit_pt(l + 64 * m);
+    r16l_transform<10, true>(v, v, x, l, ltw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) dst[l + 64 * R::end_m(s)] = v[s];
+  } else {
+    c64 va[16], vb[16];
+    // each half: its coefficient loads first, then digits and twist four
+    // points at a time
+#pragma unroll
+    for (uint32_t h = 0; h < 2; ++h) {
+      c64 *v = h == 0 ? va : vb;
+      uint64_t r[32];
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = h * 1024 + l + 64 * m;
+        r[2 * m] = ld_stream(&src[j]);
+        r[2 * m + 1] = ld_stream(&src[j + n]);
+      }
+      sched_fence();
+#pragma unroll
+      for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
+#pragma unroll
+        for (uint32_t m = m0; m < m0 + 4; ++m)
+          v[m] = cmul(twist[h * 1024 + l + 64 * m],
+                      mk((double)(int32_t)gadget_digit(r[2 * m], lvl, levels, base_log),
+                         (double)(int32_t)gadget_digit(r[2 * m + 1], lvl, levels, base_log)));
+        sched_fence();
+      }
+    }
+    r16l_2048_transform<true>(va, vb, x, l, ltw, tw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t q = l + 64 * R::end_m(s);
+      dst[q] = va[s];
+      dst[1024 + q] = vb[s];
+    }
+  }
+}
+
+template <int LOG2N>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
+                 uint64_t *__restrict__ out, uint64_t ncols, uint32_t k1, uint32_t rows,
+                 const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk,
+                 const c64 *__restrict__ untwist) {
+  using R = R16<10>;
+  constexpr uint32_t n = 1u << LOG2N;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  stage_block(tw_blk, ltw);
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;  // p * k1 + c
+  if (t >= ncols) return;
+  c64 *x = smem + kETw + wave * R::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  const uint64_t p = t / k1;
+  const uint32_t c = (uint32_t)(t % k1);
+  const c64 *sp = S + p * (uint64_t)rows * n;
+  auto mac = [&](uint32_t pos) -> c64 {
+    c64 acc = mk(0.0, 0.0);
+    for (uint32_t r = 0; r < rows; ++r) {
+      const c64 pr = cmul(keyf[((uint64_t)r * k1 + c) * n + pos], sp[(uint64_t)r * n + pos]);
+      acc = mk(acc.re + pr.re, acc.im + pr.im);
+    }
+    return acc;
+  };
+  uint64_t *o = out + t * 2ull * n;
+  auto put = [&](uint32_t j, c64 v) {
+    const c64 r = cmul(untwist[j], v);
+    st_stream(from_torus(r.re), &o[j]);
+    st_stream(from_torus(r.im), &o[j + n]);
+  };
+  if constexpr (LOG2N == 10) {
+    c64 v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      v[m] = mac(l + 64 * m);
+      sched_fence();  // bounds the loads in flight
+    }
+    r16l_transform<10, false>(v, v, x, l, ltw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) put(l + 64 * R::end_m(s), v[s]);
+  } else {
+    c64 va[16], vb[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      va[m] = mac(l + 64 * m);
+      vb[m] = mac(1024 + l + 64 * m);
+      sched_fence();  // bounds the loads in flight
+    }
+    r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t q = l + 64 * R::end_m(s);
+      put(q, va[s]);
+      put(1024 + q, vb[s]);
+    }
+  }
+}
+
+template <int LOG2N>
+hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, uint64_t batch,
+                          uint32_t k1, uint32_t levels, uint32_t base_log, const c64 *tw,
+                          const c64 *tw_blk, const c64 *twist, const c64 *untwist, c64 *S,
+                          hipStream_t s) {
+  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_extprod_rows_w<LOG2N>),
+                                         split_w_lds(8));
+  if (e == hipSuccess)
+    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_extprod_cols_w<LOG2N>),
+                                split_w_lds(8));
+  if (e != hipSuccess) return e;
+  // waves per workgroup: up to 8 units share one workgroup (one L2 fills its
+  // tables); more spread one per workgroup until they outnumber the CUs
+  // (256 on MI355X)
+  auto waves_for = [](uint64_t units) -> uint32_t {
+    if (units <= 8) return 8;
+    uint32_t w = 1;
+    while (w < 8 && units > 256ull * w) w *= 2;
+    return w;
+  };
+  const uint64_t nrows = batch * k1 * levels, ncols = batch * k1;
+  const uint32_t wr = waves_for(nrows), wc = waves_for(ncols);
+  hipLaunchKernelGGL(k_extprod_rows_w<LOG2N>, dim3((uint32_t)((nrows + wr - 1) / wr)),
+                     dim3(64 * wr), split_w_lds(wr), s, glwe, S, nrows, levels, base_log, tw,
+                     tw_blk, twist);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_extprod_cols_w<LOG2N>, dim3((uint32_t)((ncols + wc - 1) / wc)),
+                     dim3(64 * wc), split_w_lds(wc), s, keyf, S, out, ncols, k1, k1 * levels, tw,
+                     tw_blk, untwist);
+  return hipGetLastError();
+}
+
 hipError_t set_lds(const void *kernel, size_t bytes) {
   return cfft_launch::ensure_lds(kernel, bytes);
 }
@@ -331,6 +526,25 @@ hipError_t external_product_split(const Program &fwd, const Program &inv, const 
     default:
       return launch_split<16>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, S, s);
   }
+}
+
+// Split form on the wave-local kernels: Dif16 plans with n = 1024 (tw_blk =
+// the plan's r16 block) or n = 2048 and base_n = 1024 (its 1024-point block).
+hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint64_t *glwe,
+                                    uint64_t *out, uint64_t batch, uint32_t k1, uint32_t levels,
+                                    uint32_t base_log, const dc64 *tw, const dc64 *tw_blk,
+                                    const dc64 *twist, const dc64 *untwist, dc64 *scratch,
+                                    hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const c64 *kf = reinterpret_cast<const c64 *>(keyf);
+  const c64 *t = reinterpret_cast<const c64 *>(tw), *tb = reinterpret_cast<const c64 *>(tw_blk);
+  const c64 *z = reinterpret_cast<const c64 *>(twist), *u = reinterpret_cast<const c64 *>(untwist);
+  c64 *S = reinterpret_cast<c64 *>(scratch);
+  if (log2n == 10)
+    return launch_split_w<10>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
+  if (log2n == 11)
+    return launch_split_w<11>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace cfft_launch

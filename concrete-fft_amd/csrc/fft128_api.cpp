@@ -8,6 +8,7 @@
 #include "cfft.h"
 #include "host_tables.hpp"
 #include "plan.hpp"
+#include "scratch.hpp"
 
 using cfft_internal::hip_error;
 using cfft_internal::set_error;
@@ -157,13 +158,14 @@ int cfft128_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t 
   if (!a || !b || !c) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   hipStream_t s = (hipStream_t)stream;
   const size_t n = plan->n;
-  double *S = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&S), 16 * batch * n * sizeof(double), s);
-  if (e != hipSuccess) return hip_error(e, "hipMallocAsync (fft128 product scratch)");
+  cfft_internal::ScratchLease lease;
+  hipError_t e = cfft_internal::scratch_acquire(16 * batch * n * sizeof(double), s, &lease);
+  if (e != hipSuccess) return hip_error(e, "fft128 product scratch");
+  double *S = static_cast<double *>(lease.ptr);
   e = cfft_launch::exact128_polymul(ilog2(n), a, b, c, margin, batch, plan->d_tw128, S, s);
-  hipError_t e2 = hipFreeAsync(S, s);
+  hipError_t e2 = cfft_internal::scratch_release(lease, s);
   if (e != hipSuccess) return hip_error(e, "fft128 exact product");
-  return hip_error(e2, "hipFreeAsync");
+  return hip_error(e2, "fft128 product scratch release");
 }
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 #include "cfft.h"
 #include "launch.hpp"
 #include "plan.hpp"
+#include "scratch.hpp"
 
 using cfft_internal::hip_error;
 using cfft_internal::set_error;
@@ -33,6 +34,11 @@ hipError_t external_product_split(const cfft_prog::Program &fwd, const cfft_prog
                                   uint64_t batch, uint32_t k1, uint32_t levels,
                                   uint32_t base_log, const dc64 *tw, const dc64 *twist,
                                   const dc64 *untwist, dc64 *scratch, hipStream_t s);
+hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint64_t *glwe,
+                                    uint64_t *out, uint64_t batch, uint32_t k1, uint32_t levels,
+                                    uint32_t base_log, const dc64 *tw, const dc64 *tw_blk,
+                                    const dc64 *twist, const dc64 *untwist, dc64 *scratch,
+                                    hipStream_t s);
 }
 
 namespace {
@@ -68,9 +74,10 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
     return set_error(CFFT_ERROR_SIZE, "exact product supports 4 <= N <= 4096 (fft_size <= 2048)");
   hipStream_t s = (hipStream_t)stream;
   const size_t bytes = batch * 8 * plan->n * sizeof(cfft_launch::dc64);
-  void *F = nullptr;
-  hipError_t e = hipMallocAsync(&F, bytes, s);
-  if (e != hipSuccess) return hip_error(e, "hipMallocAsync (exact product scratch)");
+  cfft_internal::ScratchLease lease;
+  hipError_t e = cfft_internal::scratch_acquire(bytes, s, &lease);
+  if (e != hipSuccess) return hip_error(e, "exact product scratch");
+  void *F = lease.ptr;
   if (plan->n == 1024 && plan->d_tw_r16)  // Dif16 plan, base_n == n
     e = cfft_launch::exact_polymul_r16(a, b, c, margin, batch, plan->d_tw_r16, plan->d_twist,
                                        reinterpret_cast<cfft_launch::dc64 *>(F), s);
@@ -82,9 +89,9 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
     e = cfft_launch::exact_polymul(plan->fwd_full, plan->inv_full, a, b, c, margin, batch,
                                    plan->d_tw, plan->d_twist, plan->d_untwist,
                                    reinterpret_cast<cfft_launch::dc64 *>(F), s);
-  hipError_t e2 = hipFreeAsync(F, s);
+  hipError_t e2 = cfft_internal::scratch_release(lease, s);
   if (e != hipSuccess) return hip_error(e, "exact product");
-  return hip_error(e2, "hipFreeAsync");
+  return hip_error(e2, "exact product scratch release");
 }
 
 int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels, void *key_fourier,
@@ -128,16 +135,32 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
   if (row_blocks <= kSplitRowBlocks) {
     hipStream_t s = (hipStream_t)stream;
     const size_t bytes = row_blocks * plan->n * sizeof(cfft_launch::dc64);
-    void *S = nullptr;
-    e = hipMallocAsync(&S, bytes, s);
-    if (e != hipSuccess) return hip_error(e, "hipMallocAsync (external product scratch)");
-    e = cfft_launch::external_product_split(
-        plan->fwd_full, plan->inv_full, reinterpret_cast<const cfft_launch::dc64 *>(key_fourier),
-        glwe, out, batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
-        plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
-    hipError_t e2 = hipFreeAsync(S, s);
+    cfft_internal::ScratchLease lease;
+    e = cfft_internal::scratch_acquire(bytes, s, &lease);
+    if (e != hipSuccess) return hip_error(e, "external product scratch");
+    void *S = lease.ptr;
+    // Dif16 plans with base_n = 1024: one wave per row and per column
+    // (wave-local register-resident transforms) at n = 1024; at n = 2048
+    // only for up to 8 rows (measured: from 8 GLWEs on, two waves per row
+    // transform in the generic split form give the lower latency)
+    const cfft_launch::dc64 *blk =
+        plan->n == 1024 && plan->d_tw_r16                       ? plan->d_tw_r16
+        : plan->n == 2048 && row_blocks <= 8 && plan->d_tw_base1024 ? plan->d_tw_base1024
+                                                                     : nullptr;
+    if (blk)
+      e = cfft_launch::external_product_split_w(
+          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out,
+          batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk,
+          plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+    else
+      e = cfft_launch::external_product_split(
+          plan->fwd_full, plan->inv_full,
+          reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
+          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, plan->d_twist,
+          plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+    hipError_t e2 = cfft_internal::scratch_release(lease, s);
     if (e != hipSuccess) return hip_error(e, "external product (split)");
-    return hip_error(e2, "hipFreeAsync");
+    return hip_error(e2, "external product scratch release");
   }
   return hip_error(cfft_launch::external_product(
                        plan->fwd_full, plan->inv_full,

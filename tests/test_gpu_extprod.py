@@ -103,10 +103,11 @@ def test_external_product_errors(cfft):
                                   o, 1)
 
 
-@pytest.mark.parametrize("N", [1024, 4096])
+@pytest.mark.parametrize("N", [1024, 2048, 4096])
 def test_fused_and_split_forms_agree(cfft, oracle, N):
     # batch * k1 * levels above 1024 runs the fused one-workgroup-per-GLWE
-    # kernel; smaller calls run the split row / column kernels
+    # kernel; smaller calls run the split row / column kernels (one wave per
+    # row and column on the Dif16 plans at N = 2048 and 4096)
     k1, levels, base_log = 2, 3, 8
     rng = np.random.default_rng(N + 99)
     key = rng.integers(0, M, (k1 * levels, k1, N), dtype=np.uint64)
