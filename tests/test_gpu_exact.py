@@ -89,3 +89,33 @@ def test_exact_paths_agree_on_edge_values(cfft):
                                           a, b)
     c128 = products.exact128_polymul_u64_host(fft128.Plan(N // 2), a, b)
     assert np.array_equal(c64, c128)
+
+
+def test_exact_calls_on_two_streams_share_scratch(cfft, oracle):
+    # consecutive calls of different sizes alternate between two streams with
+    # no synchronisation between them; the products' scratch buffers are
+    # reused across calls and streams, and every result must still be exact
+    from cfft_amd import fft128, products
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    plan128 = fft128.Plan(N // 2)
+    streams = [Stream(), Stream()]
+    rng = np.random.default_rng(77)
+    calls = []
+    for i, batch in enumerate([5, 2, 7, 1, 6, 3, 4, 2]):
+        s = streams[i % 2]
+        a = rng.integers(0, M, (batch, N), dtype=np.uint64)
+        b = rng.integers(0, M, (batch, N), dtype=np.uint64)
+        da, db = DeviceBuffer.from_host(a, s), DeviceBuffer.from_host(b, s)
+        dc = DeviceBuffer((batch, N), np.uint64)
+        if i % 3 == 2:
+            products.exact128_polymul_u64(plan128, da, db, dc, batch, stream=s)
+        else:
+            products.exact_polymul_u64(plan, da, db, dc, batch, stream=s)
+        calls.append((a, b, da, db, dc))
+    for s in streams:
+        s.synchronize()
+    for a, b, _, _, dc in calls:
+        assert np.array_equal(dc.download(), oracle.schoolbook_u64(a, b))
