@@ -344,8 +344,9 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # defaults: enough warm-up launches (10 ms) for the GPU clocks to settle
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--algo", type=int, default=6, help="base FftAlgo (default Dif16)")
     ap.add_argument("--base-n", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -403,16 +404,18 @@ def main():
         products.negacyclic_polymul_u64(plan, da, db, dc, BATCH, s)
     s.synchronize()
 
-    evs = [(Event(), Event()) for _ in range(args.steps)]
+    # one HIP event pair on the launch stream brackets the timed region (a
+    # timing marker between every two launches would serialise them and
+    # count its own cost): average launch duration = elapsed / K
+    ev0, ev1 = Event(), Event()
     t0 = time.perf_counter()
+    ev0.record(s)
     for i in range(args.steps):
-        evs[i][0].record(s)
         products.negacyclic_polymul_u64(plan, da, db, dc, BATCH, s)
-        evs[i][1].record(s)
+    ev1.record(s)
     s.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [e0.elapsed_ms(e1) for e0, e1 in evs]
-    avg_ms = float(np.mean(kern_ms))
+    avg_ms = ev0.elapsed_ms(ev1) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     value = BATCH * args.steps / wall
     c_gpu = dc.download()
