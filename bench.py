@@ -100,6 +100,22 @@ UNEVEN_LOG2N = [9, 10, 11, 12, 13, 14]
 UNEVEN_WEIGHTS = [0.10, 0.20, 0.35, 0.20, 0.10, 0.05]
 
 
+def time_region(fn, stream, steps, warmup):
+    """Average ms per call of `fn` over `steps` back-to-back calls, one HIP
+    event pair on `stream` around them (throughput; no marker between calls)."""
+    from cfft_amd.device import Event
+    for _ in range(warmup):
+        fn()
+    stream.synchronize()
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
 def time_runs(fn, stream, steps, warmup):
     from cfft_amd.device import Event
     for _ in range(warmup):
@@ -138,7 +154,7 @@ def bench_uneven(steps, warmup):
         "fixed_persistent_graph": lambda: fixed.run(dx, dF, dy, s),
         "fixed_plain": lambda: fixed.run_plain(dx, dF, dy, s),
     }
-    ms = {k: float(np.median(time_runs(f, s, steps, warmup))) for k, f in forms.items()}
+    ms = {k: time_region(f, s, steps, warmup) for k, f in forms.items()}
     tuned.run(dx, dF, dy, s)
     s.synchronize()
     err = float(np.abs(dy.download() - x).max())
@@ -248,14 +264,13 @@ def bench_exact(steps, warmup):
         m1, m2 = DeviceBuffer((EXACT_BATCH,), np.float64), DeviceBuffer((EXACT_BATCH,), np.float64)
         p64 = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, min(N // 2, 1024)))
         p128 = fft128.Plan(N // 2)
-        t64 = time_runs(lambda: products.exact_polymul_u64(p64, da, db, c1, EXACT_BATCH, m1, s),
-                        s, steps, warmup)
-        t128 = time_runs(lambda: products.exact128_polymul_u64(p128, da, db, c2, EXACT_BATCH, m2,
-                                                                s), s, steps, warmup)
+        ms64 = time_region(lambda: products.exact_polymul_u64(p64, da, db, c1, EXACT_BATCH, m1, s),
+                           s, steps, warmup)
+        ms128 = time_region(lambda: products.exact128_polymul_u64(p128, da, db, c2, EXACT_BATCH,
+                                                                   m2, s), s, steps, warmup)
         r1, r2 = c1.download(), c2.download()
         k = 4
         exact = load_oracle().schoolbook_u64(a[:k], b[:k], threads=host_cores())
-        ms64, ms128 = float(np.median(t64)), float(np.median(t128))
         bytes_algo = 24 * N * EXACT_BATCH
         out.append({"N": N, "batch": EXACT_BATCH,
                     "f64_limb16": {"products_per_s": round(EXACT_BATCH / (ms64 * 1e-3), 1),
@@ -289,7 +304,7 @@ def bench_transforms(steps=50):
         batch = 8192 * 1024 // n
         plan = Plan(n, Method.UserProvided(algo, base))
         f = lambda: check(lib.cfft_fwd(plan.handle, x.dptr, batch, n, s.handle))
-        ms = float(np.median(time_runs(f, s, steps, 3)))
+        ms = time_region(f, s, steps, 10)
         gbs = 32 * n * batch / (ms * 1e-3) / 1e9
         out["n=%d %s/%d" % (n, algo.name, base)] = {
             "transforms_per_s": round(batch / (ms * 1e-3), 1), "ms": round(ms, 4),
