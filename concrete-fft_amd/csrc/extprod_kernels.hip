@@ -319,38 +319,45 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   const uint64_t p = t / k1;
   const uint32_t c = (uint32_t)(t % k1);
   const c64 *sp = S + p * (uint64_t)rows * n;
-  auto mac = [&](uint32_t pos) -> c64 {
-    c64 acc = mk(0.0, 0.0);
-    for (uint32_t r = 0; r < rows; ++r) {
-      const c64 pr = cmul(keyf[((uint64_t)r * k1 + c) * n + pos], sp[(uint64_t)r * n + pos]);
-      acc = mk(acc.re + pr.re, acc.im + pr.im);
-    }
-    return acc;
-  };
   uint64_t *o = out + t * 2ull * n;
   auto put = [&](uint32_t j, c64 v) {
     const c64 r = cmul(untwist[j], v);
     st_stream(from_torus(r.re), &o[j]);
     st_stream(from_torus(r.im), &o[j + n]);
   };
+  // Fourier multiply-accumulate row by row: each row's key and spectrum
+  // points of the lane are loaded together (one memory round trip per row,
+  // not per point); every point still sums its rows in ascending order
+  auto mac_rows = [&](c64 *acc, uint32_t base) {
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
+    for (uint32_t r = 0; r < rows; ++r) {
+      const c64 *kr = keyf + ((uint64_t)r * k1 + c) * n + base + l;
+      const c64 *sr = sp + (uint64_t)r * n + base + l;
+      c64 kv[16], sv[16];
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        kv[m] = kr[64 * m];
+        sv[m] = sr[64 * m];
+      }
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const c64 pr = cmul(kv[m], sv[m]);
+        acc[m] = mk(acc[m].re + pr.re, acc[m].im + pr.im);
+      }
+      sched_fence();  // one row's loads in flight at a time
+    }
+  };
   if constexpr (LOG2N == 10) {
     c64 v[16];
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      v[m] = mac(l + 64 * m);
-      sched_fence();  // bounds the loads in flight
-    }
+    mac_rows(v, 0);
     r16l_transform<10, false>(v, v, x, l, ltw);
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) put(l + 64 * R::end_m(s), v[s]);
   } else {
     c64 va[16], vb[16];
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      va[m] = mac(l + 64 * m);
-      vb[m] = mac(1024 + l + 64 * m);
-      sched_fence();  // bounds the loads in flight
-    }
+    mac_rows(va, 0);
+    mac_rows(vb, 1024);
     r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) {

@@ -10,6 +10,16 @@ namespace {
 
 constexpr size_t kMaxEntries = 8;
 constexpr size_t kMaxBytes = (size_t)2 << 30;
+constexpr size_t kMinEntry = (size_t)1 << 20;
+
+// entry sizes are powers of two (at least 1 MiB), so a few entries made
+// early serve later requests of slightly different sizes: a synchronous
+// hipMalloc in the middle of a serving run costs milliseconds
+size_t entry_size(size_t bytes) {
+  size_t b = kMinEntry;
+  while (b < bytes) b <<= 1;
+  return b;
+}
 
 struct Entry {
   void *ptr = nullptr;
@@ -61,17 +71,18 @@ hipError_t scratch_acquire(size_t bytes, hipStream_t s, ScratchLease *out) {
     if (best < 0 || en.bytes < pool.entries[(size_t)best].bytes) best = (int)i;
   }
   // 2. a new entry
-  if (best < 0 && pool.entries.size() < kMaxEntries && pool.bytes + bytes <= kMaxBytes) {
+  const size_t eb = entry_size(bytes);
+  if (best < 0 && pool.entries.size() < kMaxEntries && pool.bytes + eb <= kMaxBytes) {
     Entry en;
-    en.bytes = bytes;
-    e = hipMalloc(&en.ptr, bytes);
+    en.bytes = eb;
+    e = hipMalloc(&en.ptr, eb);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&en.done, hipEventDisableTiming);
     if (e != hipSuccess) {
       if (en.ptr) (void)hipFree(en.ptr);
       (void)hipGetLastError();  // fall through to the stream-ordered path
     } else {
       pool.entries.push_back(en);
-      pool.bytes += bytes;
+      pool.bytes += eb;
       best = (int)pool.entries.size() - 1;
     }
   }
