@@ -204,6 +204,24 @@ int build_f64_plan(cfft_plan_s *p) {
     CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
     CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
   }
+  if (p->algo == CFFT_DIF16 && p->base_n == 512 && n == 512) {
+    // the wave-local n = 512 transform reads the 1024-point block at column
+    // stride 2: T_1024[2 j] = T_512[j] (same sincospi argument), checked here
+    const std::vector<cfft_host::C64> t1024 = cfft_host::twiddle_table(1024);
+    bool same = true;
+    for (size_t j = 0; j < n && same; ++j)
+      same = std::memcmp(&t1024[2 * j], &tw[j], sizeof(cfft_host::C64)) == 0;
+    if (same) {
+      std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);
+      for (size_t k = 0; k < 16; ++k) {
+        for (size_t l = 0; l < 64; ++l) blk[k * 64 + l] = t1024[(k * l) % 1024];
+        for (size_t q = 0; q < 4; ++q) blk[1024 + k * 4 + q] = t1024[(16 * k * q) % 1024];
+      }
+      CFFT_HIP(hipMalloc(&p->d_tw_base1024, blk.size() * sizeof(dc64)));
+      CFFT_HIP(hipMemcpy(p->d_tw_base1024, blk.data(), blk.size() * sizeof(dc64),
+                         hipMemcpyHostToDevice));
+    }
+  }
   if (p->algo == CFFT_DIF16 && p->base_n == 1024 && n > 1024 && n <= 8192) {
     const size_t mul = n / 1024;
     std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);

@@ -200,9 +200,10 @@ __global__ void __launch_bounds__(512) k_extprod_cols(const c64 *__restrict__ ke
   run_program<E>(inv, lds, lt, T, tw, load, store);
 }
 
-// ---- split form on Dif16 plans with base_n = 1024, n = 1024 or 2048 ----
-// One wavefront per row transform and per column: the wave-local
-// register-resident Dif16 transform (n = 1024) or r16l_2048_transform
+// ---- split form on Dif16 plans, n = 512 (base 512), 1024 or 2048 (base 1024) ----
+// One wavefront per row transform and per column (two, one per 32-lane
+// half, at n = 512): the wave-local register-resident Dif16 transform
+// (n = 512, 1024) or r16l_2048_transform
 // (n = 2048: the plan's radix-2 depth pass in registers around its two
 // 1024-point chunks), W = blockDim.x / 64 waves per workgroup sharing the
 // 1024-point twiddle block in LDS.  The split form serves small batches:
@@ -244,26 +245,34 @@ k_extprod_rows_w(const uint64_t *__restrict__ glwe, c64 *__restrict__ S, uint64_
   c64 *ltw = smem;
   stage_block(tw_blk, ltw);
   __syncthreads();
+  // units (rows) per wave: 64 / T for n <= 1024 (T = n/16 lanes each), 1 at 2048
+  constexpr uint32_t T = LOG2N <= 10 ? (1u << LOG2N) / 16 : 64;
+  constexpr uint32_t G = 64 / T;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;  // (p k1 + poly) levels + lvl
-  if (t >= nrows) return;
-  c64 *x = smem + kETw + wave * R::lds_words;
-  const uint32_t l = threadIdx.x & 63u;
+  const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wave) * G;
+  if (t0 >= nrows) return;  // the whole wave is past the end
+  const uint32_t half = (threadIdx.x & 63u) / T, l = threadIdx.x % T;
+  const uint64_t t = t0 + half;  // (p k1 + poly) levels + lvl
+  const bool active = t < nrows;
+  c64 *x = smem + kETw + wave * R::lds_words + half * R16<(LOG2N <= 10 ? LOG2N : 10)>::lds_words;
   const uint32_t lvl = (uint32_t)(t % levels);
-  const uint64_t *src = glwe + (t / levels) * 2ull * n;
+  const uint64_t *src = glwe + (active ? t / levels : 0) * 2ull * n;
   c64 *dst = S + t * n;
   auto digit_pt = [&](uint32_t j) -> c64 {
     return cmul(twist[j],
                 mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
                    (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
   };
-  if constexpr (LOG2N == 10) {
+  if constexpr (LOG2N <= 10) {
+    using L = R16<LOG2N>;
     c64 v[16];
 #pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) v[m] = digit_pt(l + 64 * m);
-    r16l_transform<10, true>(v, v, x, l, ltw);
+    for (uint32_t m = 0; m < 16; ++m) v[m] = active ? digit_pt(l + T * m) : mk(0.0, 0.0);
+    r16l_transform<LOG2N, true>(v, v, x, l, ltw);
+    if (active) {
 #pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) dst[l + 64 * R::end_m(s)] = v[s];
+      for (uint32_t s = 0; s < 16; ++s) dst[l + T * L::end_m(s)] = v[s];
+    }
   } else {
     c64 va[16], vb[16];
     // each half: its coefficient loads first, then digits and twist four
@@ -311,16 +320,21 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   c64 *ltw = smem;
   stage_block(tw_blk, ltw);
   __syncthreads();
+  constexpr uint32_t T = LOG2N <= 10 ? (1u << LOG2N) / 16 : 64;
+  constexpr uint32_t G = 64 / T;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;  // p * k1 + c
-  if (t >= ncols) return;
-  c64 *x = smem + kETw + wave * R::lds_words;
-  const uint32_t l = threadIdx.x & 63u;
-  const uint64_t p = t / k1;
+  const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wave) * G;
+  if (t0 >= ncols) return;  // the whole wave is past the end
+  const uint32_t half = (threadIdx.x & 63u) / T, l = threadIdx.x % T;
+  const uint64_t t = t0 + half;  // p * k1 + c
+  const bool active = t < ncols;
+  c64 *x = smem + kETw + wave * R::lds_words + half * R16<(LOG2N <= 10 ? LOG2N : 10)>::lds_words;
+  const uint64_t p = active ? t / k1 : 0;
   const uint32_t c = (uint32_t)(t % k1);
   const c64 *sp = S + p * (uint64_t)rows * n;
   uint64_t *o = out + t * 2ull * n;
   auto put = [&](uint32_t j, c64 v) {
+    if (!active) return;
     const c64 r = cmul(untwist[j], v);
     st_stream(from_torus(r.re), &o[j]);
     st_stream(from_torus(r.im), &o[j + n]);
@@ -337,8 +351,8 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
       c64 kv[16], sv[16];
 #pragma unroll
       for (uint32_t m = 0; m < 16; ++m) {
-        kv[m] = kr[64 * m];
-        sv[m] = sr[64 * m];
+        kv[m] = kr[T * m];
+        sv[m] = sr[T * m];
       }
 #pragma unroll
       for (uint32_t m = 0; m < 16; ++m) {
@@ -348,12 +362,13 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
       sched_fence();  // one row's loads in flight at a time
     }
   };
-  if constexpr (LOG2N == 10) {
+  if constexpr (LOG2N <= 10) {
+    using L = R16<LOG2N>;
     c64 v[16];
     mac_rows(v, 0);
-    r16l_transform<10, false>(v, v, x, l, ltw);
+    r16l_transform<LOG2N, false>(v, v, x, l, ltw);
 #pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) put(l + 64 * R::end_m(s), v[s]);
+    for (uint32_t s = 0; s < 16; ++s) put(l + T * L::end_m(s), v[s]);
   } else {
     c64 va[16], vb[16];
     mac_rows(va, 0);
@@ -388,14 +403,17 @@ hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, 
     while (w < 8 && units > 256ull * w) w *= 2;
     return w;
   };
+  // units per wave: two at n = 512 (one per 32-lane half), else one
+  constexpr uint32_t G = LOG2N == 9 ? 2 : 1;
   const uint64_t nrows = batch * k1 * levels, ncols = batch * k1;
-  const uint32_t wr = waves_for(nrows), wc = waves_for(ncols);
-  hipLaunchKernelGGL(k_extprod_rows_w<LOG2N>, dim3((uint32_t)((nrows + wr - 1) / wr)),
+  const uint64_t wrows = (nrows + G - 1) / G, wcols = (ncols + G - 1) / G;  // waves needed
+  const uint32_t wr = waves_for(wrows), wc = waves_for(wcols);
+  hipLaunchKernelGGL(k_extprod_rows_w<LOG2N>, dim3((uint32_t)((wrows + wr - 1) / wr)),
                      dim3(64 * wr), split_w_lds(wr), s, glwe, S, nrows, levels, base_log, tw,
                      tw_blk, twist);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_extprod_cols_w<LOG2N>, dim3((uint32_t)((ncols + wc - 1) / wc)),
+  hipLaunchKernelGGL(k_extprod_cols_w<LOG2N>, dim3((uint32_t)((wcols + wc - 1) / wc)),
                      dim3(64 * wc), split_w_lds(wc), s, keyf, S, out, ncols, k1, k1 * levels, tw,
                      tw_blk, untwist);
   return hipGetLastError();
@@ -536,7 +554,8 @@ hipError_t external_product_split(const Program &fwd, const Program &inv, const 
 }
 
 // Split form on the wave-local kernels: Dif16 plans with n = 1024 (tw_blk =
-// the plan's r16 block) or n = 2048 and base_n = 1024 (its 1024-point block).
+// the plan's r16 block), n = 512 or n = 2048 and base_n = 1024 (the plan's
+// 1024-point block).
 hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint64_t *glwe,
                                     uint64_t *out, uint64_t batch, uint32_t k1, uint32_t levels,
                                     uint32_t base_log, const dc64 *tw, const dc64 *tw_blk,
@@ -547,6 +566,8 @@ hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint
   const c64 *t = reinterpret_cast<const c64 *>(tw), *tb = reinterpret_cast<const c64 *>(tw_blk);
   const c64 *z = reinterpret_cast<const c64 *>(twist), *u = reinterpret_cast<const c64 *>(untwist);
   c64 *S = reinterpret_cast<c64 *>(scratch);
+  if (log2n == 9)
+    return launch_split_w<9>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
   if (log2n == 10)
     return launch_split_w<10>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
   if (log2n == 11)

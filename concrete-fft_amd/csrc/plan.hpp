@@ -33,9 +33,10 @@ struct cfft_plan_s {
   // r16 (register-resident Dif16) twiddle block: T[k*l] as [k][l] (16 x n/16)
   // then T[16 k p] as [k][p] (16 x n/256)
   cfft_launch::dc64 *d_tw_r16 = nullptr;
-  // Dif16 plans with base_n = 1024 < n <= 8192: the r16 twiddle block of the
-  // 1024-point base transform (T[k l n/1024] and T[16 k p n/1024], the same
-  // values as a 1024-point plan's d_tw_r16), for the depth-form kernels
+  // Dif16 plans with base_n = 1024 < n <= 8192, and with n = base_n = 512:
+  // the r16 twiddle block of a 1024-point transform (T[k l n/1024] and
+  // T[16 k p n/1024], the same values as a 1024-point plan's d_tw_r16; at
+  // n = 512 read at column stride 2), for the wave-local kernels
   cfft_launch::dc64 *d_tw_base1024 = nullptr;
   // r8 (register-resident Dif8, n = 1024) twiddle block: T[k*l] as [k][l]
   // (8 x 128), T[8 k p] as [k][p] (8 x 16), T[64 k p] as [k][p] (8 x 2)

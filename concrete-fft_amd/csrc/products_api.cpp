@@ -139,12 +139,14 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
     e = cfft_internal::scratch_acquire(bytes, s, &lease);
     if (e != hipSuccess) return hip_error(e, "external product scratch");
     void *S = lease.ptr;
-    // Dif16 plans with base_n = 1024: one wave per row and per column
-    // (wave-local register-resident transforms) at n = 1024; at n = 2048
+    // Dif16 plans: one wave per row and per column (wave-local
+    // register-resident transforms; two per wave at n = 512) at n = 512 and
+    // 1024; at n = 2048
     // only for up to 8 rows (measured: from 8 GLWEs on, two waves per row
     // transform in the generic split form give the lower latency)
     const cfft_launch::dc64 *blk =
         plan->n == 1024 && plan->d_tw_r16                       ? plan->d_tw_r16
+        : plan->n == 512 && plan->d_tw_base1024                 ? plan->d_tw_base1024
         : plan->n == 2048 && row_blocks <= 8 && plan->d_tw_base1024 ? plan->d_tw_base1024
                                                                      : nullptr;
     if (blk)
