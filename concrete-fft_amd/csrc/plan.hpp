@@ -53,4 +53,14 @@ struct cfft_plan_s {
 namespace cfft_internal {
 int set_error(int status, const std::string &msg);
 int hip_error(hipError_t e, const char *where);
+
+// Fourier scratch the split external product of `batch` GLWEs needs (0 when
+// the batch runs fused, one workgroup per GLWE).
+size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int levels, size_t batch);
+// cfft_external_product with caller-owned scratch: `scratch` is used when it
+// holds external_product_scratch_bytes(...) bytes, else the scratch cache.
+int external_product_with_scratch(cfft_plan_s *plan, const void *key_fourier, int k1, int levels,
+                                  int base_log, const uint64_t *glwe, uint64_t *out,
+                                  size_t batch, hipStream_t stream, void *scratch,
+                                  size_t scratch_bytes);
 }  // namespace cfft_internal

@@ -60,6 +60,83 @@ uint32_t ilog2(size_t n) {
 }
 }  // namespace
 
+namespace cfft_internal {
+
+size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int levels, size_t batch) {
+  const uint64_t rows = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
+  return rows <= kSplitRowBlocks ? rows * plan->n * sizeof(cfft_launch::dc64) : 0;
+}
+
+int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k1, int levels,
+                                  int base_log, const uint64_t *glwe, uint64_t *out,
+                                  size_t batch, hipStream_t stream, void *scratch,
+                                  size_t scratch_bytes) {
+  int st = check_extprod_plan(plan);
+  if (st != CFFT_SUCCESS) return st;
+  if (k1 < 1 || levels < 1) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "k1 and levels must be >= 1");
+  if (base_log < 1 || base_log > 32 || levels * base_log > 64)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT,
+                     "gadget base_log must be in [1, 32] with levels * base_log <= 64");
+  int dev = 0, maxb = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e != hipSuccess) return hip_error(e, "device query");
+  const size_t need = cfft_launch::external_product_lds_bytes(ilog2(plan->n), (uint32_t)k1);
+  if (need > (size_t)maxb)
+    return set_error(CFFT_ERROR_SIZE, "external product: k1 Fourier accumulators of this size do "
+                                      "not fit in one workgroup's LDS");
+  if (batch == 0) return CFFT_SUCCESS;
+  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  // Small batches: one workgroup per GLWE leaves most of the device idle and
+  // runs k1 * levels + k1 transforms back to back; the split form spreads the
+  // rows and columns over workgroups (through a Fourier scratch) instead.
+  // Same accumulation order, same bits.
+  const uint64_t row_blocks = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
+  if (row_blocks <= kSplitRowBlocks) {
+    hipStream_t s = stream;
+    const size_t bytes = row_blocks * plan->n * sizeof(cfft_launch::dc64);
+    // the caller's scratch when it holds the split form's spectra, else the cache
+    cfft_internal::ScratchLease lease;
+    if (!scratch || scratch_bytes < bytes) {
+      e = cfft_internal::scratch_acquire(bytes, s, &lease);
+      if (e != hipSuccess) return hip_error(e, "external product scratch");
+    }
+    void *S = lease.ptr ? lease.ptr : scratch;
+    // Dif16 plans: one wave per row and per column (wave-local
+    // register-resident transforms; two per wave at n = 512) at n = 512 and
+    // 1024; at n = 2048
+    // only for up to 8 rows (measured: from 8 GLWEs on, two waves per row
+    // transform in the generic split form give the lower latency)
+    const cfft_launch::dc64 *blk =
+        plan->n == 1024 && plan->d_tw_r16                       ? plan->d_tw_r16
+        : plan->n == 512 && plan->d_tw_base1024                 ? plan->d_tw_base1024
+        : plan->n == 2048 && row_blocks <= 8 && plan->d_tw_base1024 ? plan->d_tw_base1024
+                                                                     : nullptr;
+    if (blk)
+      e = cfft_launch::external_product_split_w(
+          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out,
+          batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk,
+          plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+    else
+      e = cfft_launch::external_product_split(
+          plan->fwd_full, plan->inv_full,
+          reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
+          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, plan->d_twist,
+          plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+    hipError_t e2 = cfft_internal::scratch_release(lease, s);
+    if (e != hipSuccess) return hip_error(e, "external product (split)");
+    return hip_error(e2, "external product scratch release");
+  }
+  return hip_error(cfft_launch::external_product(
+                       plan->fwd_full, plan->inv_full,
+                       reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
+                       (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
+                       plan->d_twist, plan->d_untwist, stream),
+                   "external product");
+}
+
+}  // namespace cfft_internal
+
 extern "C" {
 
 int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
@@ -111,65 +188,9 @@ int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels
 int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int levels,
                           int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
                           cfft_stream_t stream) {
-  int st = check_extprod_plan(plan);
-  if (st != CFFT_SUCCESS) return st;
-  if (k1 < 1 || levels < 1) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "k1 and levels must be >= 1");
-  if (base_log < 1 || base_log > 32 || levels * base_log > 64)
-    return set_error(CFFT_ERROR_INVALID_ARGUMENT,
-                     "gadget base_log must be in [1, 32] with levels * base_log <= 64");
-  int dev = 0, maxb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&maxb, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-  if (e != hipSuccess) return hip_error(e, "device query");
-  const size_t need = cfft_launch::external_product_lds_bytes(ilog2(plan->n), (uint32_t)k1);
-  if (need > (size_t)maxb)
-    return set_error(CFFT_ERROR_SIZE, "external product: k1 Fourier accumulators of this size do "
-                                      "not fit in one workgroup's LDS");
-  if (batch == 0) return CFFT_SUCCESS;
-  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
-  // Small batches: one workgroup per GLWE leaves most of the device idle and
-  // runs k1 * levels + k1 transforms back to back; the split form spreads the
-  // rows and columns over workgroups (through a Fourier scratch) instead.
-  // Same accumulation order, same bits.
-  const uint64_t row_blocks = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
-  if (row_blocks <= kSplitRowBlocks) {
-    hipStream_t s = (hipStream_t)stream;
-    const size_t bytes = row_blocks * plan->n * sizeof(cfft_launch::dc64);
-    cfft_internal::ScratchLease lease;
-    e = cfft_internal::scratch_acquire(bytes, s, &lease);
-    if (e != hipSuccess) return hip_error(e, "external product scratch");
-    void *S = lease.ptr;
-    // Dif16 plans: one wave per row and per column (wave-local
-    // register-resident transforms; two per wave at n = 512) at n = 512 and
-    // 1024; at n = 2048
-    // only for up to 8 rows (measured: from 8 GLWEs on, two waves per row
-    // transform in the generic split form give the lower latency)
-    const cfft_launch::dc64 *blk =
-        plan->n == 1024 && plan->d_tw_r16                       ? plan->d_tw_r16
-        : plan->n == 512 && plan->d_tw_base1024                 ? plan->d_tw_base1024
-        : plan->n == 2048 && row_blocks <= 8 && plan->d_tw_base1024 ? plan->d_tw_base1024
-                                                                     : nullptr;
-    if (blk)
-      e = cfft_launch::external_product_split_w(
-          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out,
-          batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk,
-          plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
-    else
-      e = cfft_launch::external_product_split(
-          plan->fwd_full, plan->inv_full,
-          reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
-          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, plan->d_twist,
-          plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
-    hipError_t e2 = cfft_internal::scratch_release(lease, s);
-    if (e != hipSuccess) return hip_error(e, "external product (split)");
-    return hip_error(e2, "external product scratch release");
-  }
-  return hip_error(cfft_launch::external_product(
-                       plan->fwd_full, plan->inv_full,
-                       reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
-                       (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
-                       plan->d_twist, plan->d_untwist, (hipStream_t)stream),
-                   "external product");
+  return cfft_internal::external_product_with_scratch(plan, key_fourier, k1, levels, base_log,
+                                                      glwe, out, batch, (hipStream_t)stream,
+                                                      nullptr, 0);
 }
 
 }  // extern "C"

@@ -10,11 +10,13 @@ namespace {
 
 constexpr size_t kMaxEntries = 8;
 constexpr size_t kMaxBytes = (size_t)2 << 30;
-constexpr size_t kMinEntry = (size_t)1 << 20;
+// 32 MiB holds any split-form external product (at most 1024 rows of 2048
+// points) and any small exact product
+constexpr size_t kMinEntry = (size_t)32 << 20;
 
-// entry sizes are powers of two (at least 1 MiB), so a few entries made
-// early serve later requests of slightly different sizes: a synchronous
-// hipMalloc in the middle of a serving run costs milliseconds
+// entry sizes are powers of two (at least kMinEntry), so the few entries made
+// early serve every later request of the small-batch paths: a synchronous
+// hipMalloc in the middle of a serving run stalls it for milliseconds
 size_t entry_size(size_t bytes) {
   size_t b = kMinEntry;
   while (b < bytes) b <<= 1;

@@ -5,7 +5,7 @@
 // MI355X, a third of a single external product).  An entry is handed to a
 // call on stream s when its last use was on s (stream order covers it) or
 // its completion event has fired; otherwise another entry is created (a
-// power of two of at least 1 MiB; up to 8 per device and 2 GiB in all), or,
+// power of two of at least 32 MiB; up to 8 per device and 2 GiB in all), or,
 // as a last resort, s waits for the entry's event.  Requests above the cache's limits fall back
 // to hipMallocAsync / hipFreeAsync, and so do calls on a capturing stream
 // (a graph allocation node).  The cache keeps its buffers until the process

@@ -169,6 +169,7 @@ struct cfft_server_s {
   void retire(Batch &b, hipError_t err);
   bool launch_one();
   hipError_t launch(Batch &b);
+  size_t dev_scratch(const Batch &b, size_t m) const;
 };
 
 namespace {
@@ -182,6 +183,12 @@ size_t req_out_bytes(int kind, const Key *k, uint32_t log2N) {
   const size_t N = (size_t)1 << log2N;
   if (kind == CFFT_REQUEST_EXTERNAL_PRODUCT) return (size_t)k->k1 * N * 8;
   return N * 8;
+}
+
+// device bytes beyond inputs and outputs: the split external product's
+// Fourier scratch for `m` GLWEs of one key (0 when they run fused)
+size_t key_scratch(const Key &k, size_t m) {
+  return round_up(cfft_internal::external_product_scratch_bytes(k.plan, k.k1, k.levels, m));
 }
 
 int get_plan(cfft_server_s *s, uint32_t log2N, cfft_plan *out) {
@@ -224,6 +231,10 @@ cfft_server_s::~cfft_server_s() {
   if (h_pool) (void)hipHostFree(h_pool);
 }
 
+size_t cfft_server_s::dev_scratch(const Batch &b, size_t m) const {
+  return b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? key_scratch(keys[b.key], m) : 0;
+}
+
 hipError_t cfft_server_s::launch(Batch &b) {
   hipStream_t st = streams[b.slot];
   char *h_in = h_pool + b.host_off, *d_in = d_pool + b.dev_off;
@@ -246,9 +257,12 @@ hipError_t cfft_server_s::launch(Batch &b) {
   int rc;
   if (b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT) {
     const Key &k = keys[b.key];
-    rc = cfft_external_product(k.plan, k.d_fourier, k.k1, k.levels, k.base_log,
-                               reinterpret_cast<const uint64_t *>(d_in),
-                               reinterpret_cast<uint64_t *>(d_out), m, st);
+    // the split form's Fourier scratch sits in this batch's pool block, after
+    // its outputs: no allocation and no shared buffer on the serving path
+    const size_t io = round_up(b.in_bytes) + round_up(m * b.reqs[0].out_bytes);
+    rc = cfft_internal::external_product_with_scratch(
+        k.plan, k.d_fourier, k.k1, k.levels, k.base_log, reinterpret_cast<const uint64_t *>(d_in),
+        reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io);
   } else {
     cfft_plan p = plans[b.log2N].load(std::memory_order_acquire);
     rc = cfft_negacyclic_polymul_u64(p, reinterpret_cast<const uint64_t *>(d_in),
@@ -308,7 +322,8 @@ bool cfft_server_s::launch_one() {
     const Req &r = waiting[i];
     if (r.kind != b.kind || r.log2N != b.log2N || r.key != b.key) continue;
     const size_t need = round_up(in_sum + r.in_bytes) + round_up(out_sum + r.out_bytes);
-    if (need > dalloc.capacity() || need > halloc.capacity()) break;
+    const size_t dneed = need + dev_scratch(b, take.size() + 1);
+    if (dneed > dalloc.capacity() || need > halloc.capacity()) break;
     take.push_back(i);
     in_sum += r.in_bytes;
     out_sum += r.out_bytes;
@@ -316,15 +331,17 @@ bool cfft_server_s::launch_one() {
   // shrink until both pools have room (admission by free memory)
   while (!take.empty()) {
     const size_t need = round_up(in_sum) + round_up(out_sum);
+    const size_t dneed = need + dev_scratch(b, take.size());
     size_t doff = 0, hoff = 0;
-    if (dalloc.alloc(need, &doff)) {
+    if (dalloc.alloc(dneed, &doff)) {
       if (halloc.alloc(need, &hoff)) {
         b.dev_off = doff;
         b.host_off = hoff;
-        b.dev_len = b.host_len = need;
+        b.dev_len = dneed;
+        b.host_len = need;
         break;
       }
-      dalloc.release(doff, need);
+      dalloc.release(doff, dneed);
     }
     const Req &last = waiting[take.back()];
     in_sum -= last.in_bytes;
@@ -520,7 +537,8 @@ int cfft_server_submit(cfft_server s, const cfft_request *r, uint64_t *ticket) {
   q.in_bytes = req_in_bytes(q.kind, k, q.log2N);
   q.out_bytes = req_out_bytes(q.kind, k, q.log2N);
   const size_t need = round_up(q.in_bytes) + round_up(q.out_bytes);
-  if (need > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_bytes)
+  const size_t dneed = need + (k ? key_scratch(*k, 1) : 0);
+  if (dneed > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_bytes)
     return set_error(CFFT_ERROR_BUSY, "request larger than the server's memory budget");
   {
     std::lock_guard<std::mutex> g(s->mu);

@@ -300,7 +300,9 @@ int cfft_uneven_info(cfft_uneven u, size_t *njobs, size_t *ntiles, size_t *nclas
  * order and launches a batch whenever one of `streams` launch slots is free
  * and the device pool (hbm_budget_bytes, carved once from free device
  * memory) and the pinned staging pool have room for it; requests that do not
- * fit wait for running batches to release memory.  Batches on different
+ * fit wait for running batches to release memory.  A batch's device block
+ * holds its inputs, its outputs and, for a small external-product batch, the
+ * Fourier scratch of the split form, so serving allocates nothing per call.  Batches on different
  * streams overlap on the device.  Each result is bit-identical to the same
  * request computed alone (the batched kernels treat products independently). */
 typedef struct {

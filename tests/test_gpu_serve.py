@@ -83,9 +83,12 @@ def test_alone_equals_mix_equals_direct(cfft):
 def test_admission_under_small_budget(cfft):
     from cfft_amd.serve import Server
     rng = np.random.default_rng(6)
-    # room for two N = 4096 external products (64 KiB in + 64 KiB out each)
-    budget = 2 * 2 * 65536 + 4096
-    srv = Server(hbm_budget_bytes=budget, pinned_bytes=budget)
+    # room for two N = 4096 external products: 64 KiB in + 64 KiB out each in
+    # pinned staging; on the device also the split form's Fourier scratch,
+    # K1 * LEVELS spectra of 2048 c64 (192 KiB) each
+    pinned = 2 * 2 * 65536 + 4096
+    budget = 2 * (2 * 65536 + K1 * LEVELS * 2048 * 16) + 4096
+    srv = Server(hbm_budget_bytes=budget, pinned_bytes=pinned)
     keys = _setup(srv, rng, sizes=(4096,))
     reqs = [("extprod", 4096, rng.integers(0, M, (K1, 4096), dtype=np.uint64)) for _ in range(40)]
     alone = []
@@ -101,7 +104,7 @@ def test_admission_under_small_budget(cfft):
         assert np.array_equal(outs[i], alone[i])
     st = srv.stats()
     assert st["completed"] == 45
-    # a request larger than the whole budget (N = 16384 product: 384 KiB) is
+    # a request larger than the pinned budget (N = 16384 product: 384 KiB) is
     # refused at submission
     from cfft_amd import CfftPanic
     big = srv.polymul_request(np.zeros(16384, np.uint64), np.zeros(16384, np.int64))
@@ -109,6 +112,14 @@ def test_admission_under_small_budget(cfft):
         srv.submit(*big)
     assert ei.value.status == 14
     srv.close()
+    # a device pool that holds one request's input and output but not its
+    # Fourier scratch refuses the request instead of allocating beside it
+    small = Server(hbm_budget_bytes=2 * 65536 + 4096, pinned_bytes=pinned)
+    kid = small.add_key(keys[4096][1], K1, LEVELS, BASE_LOG)
+    with pytest.raises(CfftPanic) as ei:
+        small.submit(*small.extprod_request(kid, reqs[0][2]))
+    assert ei.value.status == 14
+    small.close()
 
 
 def test_poisson_replay(cfft):
