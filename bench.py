@@ -51,6 +51,7 @@ import numpy as np  # noqa: E402
 N = 2048
 BATCH = 8192
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
 # HBM bytes per 8192-product launch of the persistent Dif16 kernel from
 # rocprofv3 --pmc: FETCH_SIZE x 2 (the counter reads half of a plain copy
@@ -159,7 +160,11 @@ def bench_uneven(steps, warmup):
     s.synchronize()
     err = float(np.abs(dy.download() - x).max())
     total_N = int(np.sum(np.left_shift(1, log2_N.astype(np.int64))))
-    bytes_algo = 24 * total_N  # x read, Fourier written, round trip written (8N each)
+    # algorithmic bytes per job: a forward transform (read 8N, write 8N) and an
+    # inverse transform (read 8N, write 8N) = 32 N; the fused kernel itself
+    # moves 24 N (the inverse starts from the spectrum still in registers)
+    bytes_algo = 32 * total_N
+    bytes_fused = 24 * total_N
     best_form = min(("tuned_persistent_graph", "tuned_persistent_direct"), key=lambda k: ms[k])
     best = ms[best_form]
     plans = {}
@@ -177,7 +182,8 @@ def bench_uneven(steps, warmup):
         "roofline": {"bound": "hbm", "achieved": round(bytes_algo / (best * 1e-3) / 1e9, 2),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(bytes_algo / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-                     "bytes_per_launch": bytes_algo},
+                     "bytes_per_launch": bytes_algo, "fused_bytes_per_launch": bytes_fused,
+                     "fused_frac": round(bytes_fused / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
         "tuned_plans": plans, "autotune_s": round(tune_s, 3),
         "roundtrip_error_max": err, "schedule": tuned.info()}
 
@@ -272,10 +278,17 @@ def bench_exact(steps, warmup):
         k = 4
         exact = load_oracle().schoolbook_u64(a[:k], b[:k], threads=host_cores())
         bytes_algo = 24 * N * EXACT_BATCH
+        # FP64 flop model per product (n = N/2): 12 transforms (8 forward limb
+        # transforms, 4 inverse, one per weight) at 5 n log2 n, 10 complex
+        # multiply-adds per point (8 flop) for the weights, 12 twists (6n)
+        n = N // 2
+        flop = 12 * 5 * n * int(np.log2(n)) + 80 * n + 72 * n
         out.append({"N": N, "batch": EXACT_BATCH,
                     "f64_limb16": {"products_per_s": round(EXACT_BATCH / (ms64 * 1e-3), 1),
                                    "ms": round(ms64, 4),
                                    "hbm_frac": round(bytes_algo / (ms64 * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                                   "fp64_tflops_model": round(flop * EXACT_BATCH / (ms64 * 1e-3) / 1e12, 2),
+                                   "fp64_frac_model": round(flop * EXACT_BATCH / (ms64 * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
                                    "max_margin": float(m1.download().max())},
                     "fft128_limb32": {"products_per_s": round(EXACT_BATCH / (ms128 * 1e-3), 1),
                                       "ms": round(ms128, 4),

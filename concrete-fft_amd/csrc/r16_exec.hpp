@@ -159,6 +159,60 @@ __device__ __forceinline__ void wave_lds_fence() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// gfx950 cross-lane swaps of one f64 held in two VGPRs per lane:
+// permlane32: lanes 32-63 of `a` <-> lanes 0-31 of `b`;
+// permlane16: odd 16-lane rows of `a` <-> even rows of `b`.
+__device__ __forceinline__ void swap_rows32(double &a, double &b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane32_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  b = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+}
+__device__ __forceinline__ void swap_rows16(double &a, double &b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)ua, (uint32_t)ub, false, false);
+  const auto hi =
+      __builtin_amdgcn_permlane16_swap((uint32_t)(ua >> 32), (uint32_t)(ub >> 32), false, false);
+  a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  b = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+}
+
+// The second exchange of a wave-local Dif16 transform (n = 512 or 1024,
+// T = n / 16 lanes per transform, lane l = q + 16 p with q < 16; p is the
+// lane's 16-lane row within its transform) in registers.  Through LDS it
+// writes slot k of lane (q, p) at q + 256 p + 16 k and reads slot
+// i REND + k of lane (q, p) from l + T (i + NB k); matching the two indices,
+//   n = 1024 (REND = 4): slot 4 i + k of row p <- slot 4 i + p of row k,
+//   n = 512  (REND = 2): slot 2 i + k of row p <- slot 2 i + p of row k,
+// with q unchanged: a transpose of the row index against the low register
+// bits, which the row swaps perform without LDS.
+template <int LOG2N>
+__device__ __forceinline__ void r16_exchange2_rows(c64 *v) {
+  // (n = 256 has no second exchange: its callers return before it)
+  if constexpr (LOG2N == 10) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      c64 *g = v + 4 * i;
+      swap_rows32(g[0].re, g[2].re);
+      swap_rows32(g[0].im, g[2].im);
+      swap_rows32(g[1].re, g[3].re);
+      swap_rows32(g[1].im, g[3].im);
+      swap_rows16(g[0].re, g[1].re);
+      swap_rows16(g[0].im, g[1].im);
+      swap_rows16(g[2].re, g[3].re);
+      swap_rows16(g[2].im, g[3].im);
+    }
+  } else if constexpr (LOG2N == 9) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      swap_rows16(v[2 * i].re, v[2 * i + 1].re);
+      swap_rows16(v[2 * i].im, v[2 * i + 1].im);
+    }
+  }
+}
+
 template <bool FWD>
 __device__ __forceinline__ void twiddle16_x2(c64 *va, c64 *vb, const c64 *tw_row0,
                                              uint32_t row_stride, uint32_t col) {
@@ -217,7 +271,7 @@ __device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_
   if (NV == 2) twiddle16_x2<FWD>(va, vb, tw, S::T, l);
   else twiddle16_x1<FWD>(va, tw, S::T, l);
   // exchange 1
-  const uint32_t p = l >> 4, q = l & 15;
+  const uint32_t p = l >> 4;
 #pragma unroll
   for (int t = 0; t < NV; ++t) {
     c64 *v = t == 0 ? va : vb;
@@ -228,27 +282,16 @@ __device__ __forceinline__ void r16w_transform(c64 *va, c64 *vb, c64 *x, uint32_
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
   }
+  wave_lds_fence();  // last use of x
   // core, stride 16
   butterfly<16, FWD>(va);
   if (NV == 2) butterfly<16, FWD>(vb);
   sched_fence();
   if (NV == 2) twiddle16_x2<FWD>(va, vb, tw + 16 * S::T, S::T / 16, p);
   else twiddle16_x1<FWD>(va, tw + 16 * S::T, S::T / 16, p);
-  // exchange 2
-#pragma unroll
-  for (int t = 0; t < NV; ++t) {
-    c64 *v = t == 0 ? va : vb;
-    wave_lds_fence();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
-    wave_lds_fence();
-#pragma unroll
-    for (uint32_t i = 0; i < S::NB; ++i)
-#pragma unroll
-      for (uint32_t k = 0; k < S::REND; ++k)
-        v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
-  }
-  wave_lds_fence();
+  // exchange 2, in registers
+  r16_exchange2_rows<10>(va);
+  if (NV == 2) r16_exchange2_rows<10>(vb);
   // end butterflies (radix 4)
 #pragma unroll
   for (uint32_t i = 0; i < S::NB; ++i) {
@@ -295,7 +338,7 @@ __device__ __forceinline__ void r16l_transform(c64 *va, c64 *vb, c64 *x, uint32_
     if (NV == 2) butterfly<16, FWD>(vb);
     return;
   }
-  const uint32_t p = l >> 4, q = l & 15;
+  const uint32_t p = l >> 4;
 #pragma unroll
   for (int t = 0; t < NV; ++t) {
     c64 *v = t == 0 ? va : vb;
@@ -306,25 +349,15 @@ __device__ __forceinline__ void r16l_transform(c64 *va, c64 *vb, c64 *x, uint32_
 #pragma unroll
     for (int k = 0; k < 16; ++k) v[k] = x[r16_pad(l + S::T * (uint32_t)k)];
   }
+  wave_lds_fence();  // last use of x
   butterfly<16, FWD>(va);
   if (NV == 2) butterfly<16, FWD>(vb);
   sched_fence();
   if (NV == 2) twiddle16_x2<FWD>(va, vb, tw + 16 * 64, 4, p * cs);
   else twiddle16_x1<FWD>(va, tw + 16 * 64, 4, p * cs);
-#pragma unroll
-  for (int t = 0; t < NV; ++t) {
-    c64 *v = t == 0 ? va : vb;
-    wave_lds_fence();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) x[r16_pad(q + 256 * p + 16 * (uint32_t)k)] = v[k];
-    wave_lds_fence();
-#pragma unroll
-    for (uint32_t i = 0; i < S::NB; ++i)
-#pragma unroll
-      for (uint32_t k = 0; k < S::REND; ++k)
-        v[i * S::REND + k] = x[r16_pad(l + S::T * (i + S::NB * k))];
-  }
-  wave_lds_fence();
+  // exchange 2, in registers
+  r16_exchange2_rows<LOG2N>(va);
+  if (NV == 2) r16_exchange2_rows<LOG2N>(vb);
 #pragma unroll
   for (uint32_t i = 0; i < S::NB; ++i) {
     if constexpr (S::REND > 1) {
