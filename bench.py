@@ -56,8 +56,8 @@ BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
 # HBM bytes per 8192-product launch of the persistent Dif16 kernel from
 # rocprofv3 --pmc: FETCH_SIZE x 2 (the counter reads half of a plain copy
 # kernel's known read size on this platform) + WRITE_SIZE (exact for a copy):
-# (131431 x 2 + 131072) KiB.
-TRAFFIC_BYTES = (131431 * 2 + 131072) * 1024
+# (131400 x 2 + 131078) KiB.
+TRAFFIC_BYTES = (131400 * 2 + 131078) * 1024
 
 
 def host_cores():

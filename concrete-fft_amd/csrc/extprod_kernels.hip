@@ -394,11 +394,15 @@ hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, 
     e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_extprod_cols_w<LOG2N>),
                                 split_w_lds(8));
   if (e != hipSuccess) return e;
-  // waves per workgroup: up to 8 units share one workgroup (one L2 fills its
+  // waves per workgroup: a few waves share one workgroup (one L2 fills its
   // tables); more spread one per workgroup until they outnumber the CUs
-  // (256 on MI355X)
-  auto waves_for = [](uint64_t units) -> uint32_t {
-    if (units <= 8) return 8;
+  // (256 on MI355X).  Measured: up to 8 row waves (decompose + forward) and
+  // up to 4 column waves (6-row MAC + inverse) gain from sharing; 8 column
+  // waves in one workgroup took 41 us against 32 us spread (N = 1024,
+  // batch 8), while 6 row waves spread took 30 us against 28 us shared
+  // (N = 2048, batch 1).
+  auto waves_for = [](uint64_t units, uint64_t share) -> uint32_t {
+    if (units <= share) return 8;
     uint32_t w = 1;
     while (w < 8 && units > 256ull * w) w *= 2;
     return w;
@@ -407,7 +411,7 @@ hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, 
   constexpr uint32_t G = LOG2N == 9 ? 2 : 1;
   const uint64_t nrows = batch * k1 * levels, ncols = batch * k1;
   const uint64_t wrows = (nrows + G - 1) / G, wcols = (ncols + G - 1) / G;  // waves needed
-  const uint32_t wr = waves_for(wrows), wc = waves_for(wcols);
+  const uint32_t wr = waves_for(wrows, 8), wc = waves_for(wcols, 4);
   hipLaunchKernelGGL(k_extprod_rows_w<LOG2N>, dim3((uint32_t)((wrows + wr - 1) / wr)),
                      dim3(64 * wr), split_w_lds(wr), s, glwe, S, nrows, levels, base_log, tw,
                      tw_blk, twist);
