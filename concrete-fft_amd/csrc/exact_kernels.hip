@@ -128,9 +128,13 @@ __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
 
 // ---- register-resident form for Dif16 plans with n = 1024 (N = 2048) ----
 // Forward: one wave per (product, limb index i) transforms a_i and b_i
-// together (wave-local Dif16 schedule, twiddles shared); spectra go to the
-// same scratch layout as above.  8 waves per workgroup share the twiddle
-// block staged in LDS.
+// together (wave-local Dif16 schedule, twiddles shared); 8 waves per
+// workgroup (two products) share the twiddle block staged in LDS.  The
+// product's four waves then meet in their exchange buffers, half of the
+// slots at a time, and each lane forms the four weight spectra
+// W_k = sum_{i<=k} F[a_i] F[b_{k-i}] (i ascending) of one point of each
+// product: the scratch holds 4 spectra per product, written once and read
+// once by the inverse.
 constexpr uint32_t kXW = 8;
 constexpr uint32_t kXTw = 16 * 64 + 16 * 4;
 constexpr size_t kXLds = (kXTw + kXW * R16<10>::lds_words) * sizeof(c64);
@@ -147,36 +151,70 @@ k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t u = (uint64_t)blockIdx.x * kXW + wave;  // 4 p + i
-  if (u >= batch * 4) return;
+  const bool active = u < batch * 4;  // every wave reaches the barriers below
   c64 *x = smem + kXTw + wave * S::lds_words;
   const uint32_t l = threadIdx.x & 63u;
   const uint64_t p = u >> 2;
   const uint32_t li = (uint32_t)(u & 3);
-  if (li == 0 && l == 0 && margin) margin[p] = 0.0;
-  const uint64_t *a = A + p * (2 * n);
-  const uint64_t *b = B + p * (2 * n);
   c64 fa[16], fb[16];
+  if (active) {
+    if (li == 0 && l == 0 && margin) margin[p] = 0.0;
+    const uint64_t *a = A + p * (2 * n);
+    const uint64_t *b = B + p * (2 * n);
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) {
-    const uint32_t j = l + S::T * m;
-    const c64 z = twist[j];
-    fa[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&a[j]), li),
-                       (double)(int32_t)limb16(ld_stream(&a[j + n]), li)));
-    fb[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&b[j]), li),
-                       (double)(int32_t)limb16(ld_stream(&b[j + n]), li)));
+    for (uint32_t m = 0; m < 16; ++m) {
+      const uint32_t j = l + S::T * m;
+      const c64 z = twist[j];
+      fa[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&a[j]), li),
+                         (double)(int32_t)limb16(ld_stream(&a[j + n]), li)));
+      fb[m] = cmul(z, mk((double)(int32_t)limb16(ld_stream(&b[j]), li),
+                         (double)(int32_t)limb16(ld_stream(&b[j + n]), li)));
+    }
+    r16w_transform<true, 2>(fa, fb, x, l, ltw);
   }
-  r16w_transform<true, 2>(fa, fb, x, l, ltw);
-  c64 *oa = F + (8 * p + li) * n, *ob = F + (8 * p + 4 + li) * n;
+  // weights: the buffer of wave 4 pp + i holds, for the half's slots s,
+  // a_i's slot s of lane l at (s - 8 h) * 64 + l and b_i's 512 words later;
+  // thread t forms the four weights of point t of the half of both products
+  const c64 *xs = smem + kXTw;
+  const uint32_t t = threadIdx.x;
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) {
-    const uint32_t pos = l + S::T * S::end_m(s);
-    oa[pos] = fa[s];
-    ob[pos] = fb[s];
+  for (uint32_t h = 0; h < 2; ++h) {
+    wave_lds_fence();
+    if (active) {  // (an inactive wave's buffer belongs to no product)
+#pragma unroll
+      for (uint32_t s = 0; s < 8; ++s) {
+        x[s * 64 + l] = fa[8 * h + s];
+        x[512 + s * 64 + l] = fb[8 * h + s];
+      }
+    }
+    __syncthreads();
+    const uint32_t q = (t & 63) + S::T * S::end_m(8 * h + (t >> 6));
+#pragma unroll
+    for (uint32_t pp = 0; pp < 2; ++pp) {
+      const uint64_t pq = (uint64_t)blockIdx.x * 2 + pp;
+      if (pq < batch) {
+        c64 wa[4], wb[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+          wa[i] = xs[(4 * pp + i) * S::lds_words + t];
+          wb[i] = xs[(4 * pp + i) * S::lds_words + 512 + t];
+        }
+        c64 *W = F + pq * 4ull * n;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          c64 d = cmul(wa[0], wb[k]);
+#pragma unroll
+          for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(wa[i], wb[k - i]));
+          W[k * n + q] = d;
+        }
+      }
+    }
+    __syncthreads();  // the buffers are rewritten for the second half
   }
 }
 
-// Inverse: 4 waves per product, wave k computes 2^(16 k) D_k (its weight's
-// Fourier sum, inverse transform, rounding, shift) into its own LDS buffer;
+// Inverse: 4 waves per product, wave k computes 2^(16 k) D_k (inverse
+// transform of W_k, rounding, shift) into its own LDS buffer;
 // after a barrier the product's first wave sums the four buffers mod 2^64.
 __global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *margin,
@@ -193,19 +231,12 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
   const bool active = p < batch;  // every wave reaches the barrier below
   c64 *x = smem + kXTw + wave * S::lds_words;
   const uint32_t l = threadIdx.x & 63u;
-  const c64 *fp = F + (active ? p : 0) * 8ull * n;
+  const c64 *wk = F + (active ? p : 0) * 4ull * n + k * n;  // weight spectrum W_k
   const double inv_n = 1.0 / (double)n;
   double worst = 0.0;
   c64 v[16];
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) {
-    const uint32_t pos = l + S::T * m;
-    c64 d = cmul(fp[pos], fp[(4 + k) * n + pos]);
-    for (uint32_t i = 1; i <= k; ++i)
-      d = cadd(d, cmul(fp[i * n + pos], fp[(4 + k - i) * n + pos]));
-    v[m] = d;
-    if ((m & 3) == 3) sched_fence();  // bounds the loads in flight
-  }
+  for (uint32_t m = 0; m < 16; ++m) v[m] = wk[l + S::T * m];
   r16w_transform<false, 1>(v, v, x, l, ltw);
   uint64_t part[32];
   const c64 *tws = launder(twist);
@@ -249,12 +280,17 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
 // ---- wave-local form for Dif16 plans with n = 2048, base_n = 1024 (N = 4096) ----
 // One 2048-point transform per wavefront (r16l_2048_transform: the plan's
 // radix-2 depth pass in registers around two 1024-point base transforms run
-// together).  Forward: one wave per (product, limb transform), 8 waves per
-// workgroup.  Inverse: one wave per (product, weight k); the rounded,
-// shifted parts of the four weights meet in LDS, chunk 0's positions first,
-// then chunk 1's, and each of the product's waves adds and stores a quarter
-// of them.  (Both kernels keep a few dozen VGPRs in scratch: the paired
-// 1024-point transforms alone take 240 of the 256 available.)
+// together).  Forward: one workgroup per product, one wave per limb
+// transform; the eight limb spectra (same slot layout in every wave) meet in
+// the waves' exchange buffers, one half at a time, and each lane forms the
+// four weight spectra W_k = sum_{i<=k} F[a_i] F[b_{k-i}] (i ascending) of
+// two points, so the scratch holds 4 spectra per product, written once and
+// read once.  Inverse: one wave per (product, weight k) reading W_k; the
+// rounded, shifted parts of the four weights meet in LDS, chunk 0's
+// positions first, then chunk 1's, and each of the product's waves adds and
+// stores a quarter of them.  (The forward keeps a few dozen VGPRs in
+// scratch: the paired 1024-point transforms alone take 240 of the 256
+// available.)
 constexpr uint32_t kWN = 2048;
 
 __global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -301,12 +337,39 @@ k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
     }
   }
   r16l_2048_transform<true>(va, vb, x, l, ltw, tw);
-  c64 *out = F + u * n;
+  // weights: buffer w holds limb spectrum w of the product (a_w for w < 4,
+  // b_{w-4} after), slot s of lane l at s * 64 + l; thread t forms the four
+  // weights of buffer positions t and t + 512 (every wave of the workgroup
+  // belongs to product p: the grid has one workgroup per product)
+  c64 *W = F + p * 4ull * n;
+  const c64 *xs = smem + kXTw;
+  const uint32_t t = threadIdx.x;
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) {
-    const uint32_t q = l + 64 * S::end_m(s);
-    out[q] = va[s];
-    out[1024 + q] = vb[s];
+  for (uint32_t h = 0; h < 2; ++h) {
+    const c64 *v = h == 0 ? va : vb;
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) x[s * 64 + l] = v[s];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t e = 0; e < 2; ++e) {
+      const uint32_t idx = t + 512 * e;
+      const uint32_t q = h * 1024 + (idx & 63) + 64 * S::end_m(idx >> 6);
+      c64 fa[4], fb[4];
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) {
+        fa[i] = xs[i * S::lds_words + idx];
+        fb[i] = xs[(4 + i) * S::lds_words + idx];
+      }
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) {
+        c64 d = cmul(fa[0], fb[k]);
+#pragma unroll
+        for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(fa[i], fb[k - i]));
+        W[k * n + q] = d;
+      }
+    }
+    __syncthreads();  // the buffers are rewritten for the second half
   }
 }
 
@@ -326,20 +389,13 @@ k_exact_inv_w2(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *marg
   const bool active = p < batch;  // every wave reaches the barriers below
   c64 *x = smem + kXTw + wave * S::lds_words;
   const uint32_t l = threadIdx.x & 63u;
-  const c64 *fp = F + (active ? p : 0) * 8ull * n;
+  const c64 *wk = F + (active ? p : 0) * 4ull * n + k * n;  // weight spectrum W_k
   c64 va[16], vb[16];
 #pragma unroll
   for (uint32_t m = 0; m < 16; ++m) {
     const uint32_t q = l + 64 * m;
-    c64 da = cmul(fp[q], fp[(4 + k) * n + q]);
-    c64 db = cmul(fp[1024 + q], fp[(4 + k) * n + 1024 + q]);
-    for (uint32_t i = 1; i <= k; ++i) {
-      da = cadd(da, cmul(fp[i * n + q], fp[(4 + k - i) * n + q]));
-      db = cadd(db, cmul(fp[i * n + 1024 + q], fp[(4 + k - i) * n + 1024 + q]));
-    }
-    va[m] = da;
-    vb[m] = db;
-    sched_fence();  // bounds the loads in flight
+    va[m] = wk[q];
+    vb[m] = wk[1024 + q];
   }
   r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
   // the transform's last LDS reads are behind its final wave fence; the

@@ -27,6 +27,23 @@ def test_exact_bit_exact_random(cfft, oracle, N, algo, base_n):
     print("N=%d max rounding margin %.3g" % (N, margin.max()))
 
 
+@pytest.mark.parametrize("N", [2048, 4096])
+@pytest.mark.parametrize("batch", [1, 3, 5])
+def test_exact_weight_spectra_odd_batches(cfft, oracle, N, batch):
+    # Dif16 plans with base 1024 form the four weight spectra inside the
+    # forward kernel (two products per workgroup at N = 2048, so an odd
+    # batch leaves half of the last workgroup without a product)
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(1000 * batch + N)
+    a = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    b = rng.integers(0, M, (batch, N), dtype=np.uint64)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c, margin = products.exact_polymul_u64_host(plan, a, b, with_margin=True)
+    assert np.array_equal(c, oracle.schoolbook_u64(a, b))
+    assert margin.max() < 0.5 and margin.min() >= 0.0
+
+
 def test_exact_edge_values(cfft, oracle):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
