@@ -50,6 +50,12 @@ hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint
                        uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s);
 
 
+// Dif16 plans with n = base_n = 1024: two transforms per wavefront
+// (r16w_transform), 8 wavefronts per workgroup; tw_r16 is the plan's r16
+// twiddle block.
+hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                           const dc64 *tw_r16, hipStream_t s);
+
 // Dif16 plans with n = 2048 and base_n = 1024: one transform per wavefront
 // (r16l_2048_transform); tw is the plan's table, tw_blk its 1024-point block.
 hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,

@@ -321,9 +321,67 @@ k_w2048_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
+// ---- Dif16 plan with n = base_n = 1024: NV transforms per wavefront ----
+// r16w_transform (twiddle reads shared by the NV transforms), 8 waves per
+// workgroup sharing the twiddle block staged in LDS.  In place: each wave
+// reads its transforms before writing them.  A wave whose second transform
+// would lie past the batch repeats its first one and stores it once.
+template <bool FWD, int NV>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_w1024_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
+                  const c64 *__restrict__ tw_r16) {
+  using S = R16<10>;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t t0 = ((uint64_t)blockIdx.x * kPW + wave) * NV;
+  if (t0 >= batch) return;
+  const bool two = NV == 2 && t0 + 1 < batch;
+  c64 *x = smem + kTwWords + wave * S::lds_words;
+  const uint32_t l = threadIdx.x & 63u;
+  c64 *d0 = data + t0 * stride;
+  c64 *d1 = two ? d0 + stride : d0;
+  c64 va[16], vb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    va[m] = ld_stream(&d0[l + 64 * m]);
+    if (NV == 2) vb[m] = ld_stream(&d1[l + 64 * m]);
+  }
+  r16w_transform<FWD, NV>(va, vb, x, l, ltw);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t q = l + 64 * S::end_m(s);
+    st_stream(va[s], &d0[q]);
+    if (NV == 2 && two) st_stream(vb[s], &d1[q]);
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
+
+hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                           const dc64 *tw_r16, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  constexpr int NV = 2;
+  const void *k = fwd ? reinterpret_cast<const void *>(k_w1024_transform<true, NV>)
+                      : reinterpret_cast<const void *>(k_w1024_transform<false, NV>);
+  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  if (e != hipSuccess) return e;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw_r16);
+  const uint64_t per_wg = (uint64_t)kPW * NV;
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
+  if (fwd)
+    hipLaunchKernelGGL((k_w1024_transform<true, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+                       batch, stride, t);
+  else
+    hipLaunchKernelGGL((k_w1024_transform<false, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+                       batch, stride, t);
+  return hipGetLastError();
+}
 
 hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
                            const dc64 *tw_blk, hipStream_t s) {
@@ -375,7 +433,7 @@ hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, u
   if (batch == 0) return hipSuccess;
   switch (log2n) {
     case 9: return launch_transform<9>(fwd, data, batch, stride, tw, s);
-    case 10: return launch_transform<10>(fwd, data, batch, stride, tw, s);
+    case 10: return w1024_transform(fwd, data, batch, stride, tw, s);
     case 11: return launch_transform<11>(fwd, data, batch, stride, tw, s);
     case 12: return launch_transform<12>(fwd, data, batch, stride, tw, s);
     default: return hipErrorInvalidValue;
