@@ -72,6 +72,7 @@ _SIGS = [
     ("cfft_event_synchronize", c_int, [c_vp]),
     ("cfft_event_elapsed_ms", c_int, [c_vp, c_vp, ctypes.POINTER(ctypes.c_float)]),
     ("cfft_ordered_plan_create", c_int, [c_size, c_int, c_int, c_dbl, ctypes.POINTER(c_vp)]),
+    ("cfft_ordered_plan_create_large", c_int, [c_size, c_int, c_int, c_dbl, ctypes.POINTER(c_vp)]),
     ("cfft_unordered_plan_create", c_int,
      [c_size, c_int, c_int, c_size, c_dbl, ctypes.POINTER(c_vp)]),
     ("cfft128_plan_create", c_int, [c_size, ctypes.POINTER(c_vp)]),

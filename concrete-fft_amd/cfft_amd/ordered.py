@@ -110,10 +110,12 @@ class _PlanBase:
 class Plan(_PlanBase):
     """ordered::Plan (src/ordered.rs:186-373): standard order in and out, n <= 1024."""
 
-    def __init__(self, n, method):
+    def __init__(self, n, method, large=False):
+        """large=True admits 2^11 <= n <= 2^13 (cfft_ordered_plan_create_large):
+        the reference's ordered recursion beyond its n < 2^11 assertion."""
         h = ctypes.c_void_p()
-        check(lib.cfft_ordered_plan_create(n, method.kind, int(method.algo), method.seconds,
-                                           ctypes.byref(h)))
+        create = lib.cfft_ordered_plan_create_large if large else lib.cfft_ordered_plan_create
+        check(create(n, method.kind, int(method.algo), method.seconds, ctypes.byref(h)))
         self.handle = h.value
 
     def algo(self):

@@ -561,6 +561,25 @@ int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seco
   return create_f64(cfft_plan_s::ORDERED, n, algo, n, out);
 }
 
+int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measure_seconds,
+                                   cfft_plan *out) {
+  if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");
+  *out = nullptr;
+  if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
+  // the whole transform runs in one workgroup's LDS chunk
+  if (ilog2(n) > cfft_launch::kLdsMaxLog2)
+    return set_error(CFFT_ERROR_SIZE, "ordered plans run up to n = 2^13 on the device");
+  size_t base = n;
+  if (method == CFFT_METHOD_MEASURE) {
+    int rc = measure(cfft_plan_s::ORDERED, n, measure_seconds, &algo, &base);
+    if (rc) return rc;
+  } else if (method != CFFT_METHOD_USER_PROVIDED) {
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown method");
+  }
+  if (algo_radix(algo) == 0) return set_error(CFFT_ERROR_INVALID_ALGO, "unknown FftAlgo");
+  return create_f64(cfft_plan_s::ORDERED, n, algo, n, out);
+}
+
 int cfft_unordered_plan_create(size_t n, int method, int base_algo, size_t base_n,
                                double measure_seconds, cfft_plan *out) {
   if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");

@@ -105,6 +105,16 @@ int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms);
  * the device for about `measure_seconds` each (Method::Measure(Duration)). */
 int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seconds,
                              cfft_plan *out);
+/* The same ordered plan for 2^11 <= n <= 2^13 (and any smaller n).  The
+ * reference documents the ordered plan up to 2^16 (src/ordered.rs:185) but
+ * Plan::new asserts n < 2^11 (src/ordered.rs:244), so no reference output
+ * exists at these sizes: the plan runs the reference's ordered recursion
+ * (Stockham cores of the chosen radix, same twiddle table) extended to n,
+ * checked against the closed-form DFT.  n above 2^13 (one workgroup's LDS)
+ * returns CFFT_ERROR_SIZE.  cfft_ordered_plan_create keeps the reference's
+ * assertion. */
+int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measure_seconds,
+                                   cfft_plan *out);
 /* unordered::Plan::new(n, method), src/unordered.rs:659-747: n a power of
  * two; base_n a power of two with base_n <= n, base_n <= 2^10 and
  * base_n >= 32 unless base_n == n.  MEASURE picks (base_algo, base_n) on the

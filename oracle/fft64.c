@@ -573,8 +573,8 @@ int or_unordered_inv_batch(size_t n, int base_algo, size_t base_n, c64 *buf, siz
   or_uplan_free(&p);
   return 0;
 }
-static int ordered_batch(size_t n, int algo, c64 *buf, size_t batch, int fwd) {
-  if (n == 0 || (n & (n - 1)) != 0 || n > 1024 || or_algo_radix(algo) == 0) return -1;
+static int ordered_batch_max(size_t n, int algo, c64 *buf, size_t batch, int fwd, size_t nmax) {
+  if (n == 0 || (n & (n - 1)) != 0 || n > nmax || or_algo_radix(algo) == 0) return -1;
   c64 *tw = (c64 *)calloc(2 * n, sizeof(c64));
   c64 *twi = (c64 *)calloc(2 * n, sizeof(c64));
   c64 *scratch = (c64 *)malloc(n * sizeof(c64));
@@ -586,5 +586,12 @@ static int ordered_batch(size_t n, int algo, c64 *buf, size_t batch, int fwd) {
   free(tw); free(twi); free(scratch);
   return 0;
 }
-int or_ordered_fwd_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch(n, algo, buf, batch, 1); }
-int or_ordered_inv_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch(n, algo, buf, batch, 0); }
+/* Plan::new asserts n < 2^11 (src/ordered.rs:243-244) */
+int or_ordered_fwd_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 1, 1024); }
+int or_ordered_inv_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 0, 1024); }
+/* The same recursion and twiddle tables without that assert, up to n = 2^13
+ * (the plan's documented range reaches 2^16, src/ordered.rs:185).  No output
+ * of the reference pins these sizes: they are checked against the closed-form
+ * DFT and by round trips. */
+int or_ordered_fwd_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 1, 8192); }
+int or_ordered_inv_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 0, 8192); }

@@ -77,6 +77,8 @@ int or_unordered_fwd_batch(size_t n, int base_algo, size_t base_n, or_c64 *buf, 
 int or_unordered_inv_batch(size_t n, int base_algo, size_t base_n, or_c64 *buf, size_t batch);
 int or_ordered_fwd_batch(size_t n, int algo, or_c64 *buf, size_t batch);
 int or_ordered_inv_batch(size_t n, int algo, or_c64 *buf, size_t batch);
+int or_ordered_fwd_batch_large(size_t n, int algo, or_c64 *buf, size_t batch);
+int or_ordered_inv_batch_large(size_t n, int algo, or_c64 *buf, size_t batch);
 
 /* ---- rand 0.8 StdRng (ChaCha12, rand_chacha 0.3) seeded with
  *      rand_core 0.6 seed_from_u64 (PCG32 expansion); gen_range(0.0..1.0) ---- */

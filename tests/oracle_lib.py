@@ -29,6 +29,8 @@ class Oracle:
             "or_unordered_inv_batch": (c_int, [c_size, c_int, c_size, c_vp, c_size]),
             "or_ordered_fwd_batch": (c_int, [c_size, c_int, c_vp, c_size]),
             "or_ordered_inv_batch": (c_int, [c_size, c_int, c_vp, c_size]),
+            "or_ordered_fwd_batch_large": (c_int, [c_size, c_int, c_vp, c_size]),
+            "or_ordered_inv_batch_large": (c_int, [c_size, c_int, c_vp, c_size]),
             "or_stdrng_fill_f64_01": (None, [ctypes.c_uint64, c_vp, c_size]),
             "or_bit_rev_twice": (c_size, [ctypes.c_uint, ctypes.c_uint, c_size]),
             "or_bit_rev_twice_inv": (c_size, [ctypes.c_uint, ctypes.c_uint, c_size]),
@@ -85,13 +87,15 @@ class Oracle:
     def ordered_fwd(self, z, algo):
         z = np.ascontiguousarray(z, dtype=np.complex128).copy()
         n = z.shape[-1]
-        assert self.L.or_ordered_fwd_batch(n, int(algo), z.ctypes.data, z.size // n) == 0
+        f = self.L.or_ordered_fwd_batch if n <= 1024 else self.L.or_ordered_fwd_batch_large
+        assert f(n, int(algo), z.ctypes.data, z.size // n) == 0
         return z
 
     def ordered_inv(self, z, algo):
         z = np.ascontiguousarray(z, dtype=np.complex128).copy()
         n = z.shape[-1]
-        assert self.L.or_ordered_inv_batch(n, int(algo), z.ctypes.data, z.size // n) == 0
+        f = self.L.or_ordered_inv_batch if n <= 1024 else self.L.or_ordered_inv_batch_large
+        assert f(n, int(algo), z.ctypes.data, z.size // n) == 0
         return z
 
     def stdrng_f64(self, seed, count):

@@ -54,6 +54,16 @@ def test_ordered_rejects_too_large():
     assert "n.trailing_zeros() < 11" in str(e.value)
 
 
+@pytest.mark.parametrize("n,status", [(16384, 2), (3072, 1)])
+def test_ordered_large_limits(n, status):
+    # the extended entry point stops where one workgroup's LDS ends (2^13)
+    import cfft_amd
+    from cfft_amd.ordered import FftAlgo, Method, Plan
+    with pytest.raises(cfft_amd.CfftPanic) as e:
+        Plan(n, Method.UserProvided(FftAlgo.Dif4), large=True)
+    assert e.value.status == status
+
+
 @pytest.mark.parametrize("n,base_n,msg", [
     (1024, 48, "base_n.is_power_of_two()"),
     (1024, 2048, "base_n <= n"),

@@ -49,6 +49,32 @@ def test_ordered_bit_exact(cfft, oracle, algo):
             assert np.max(np.abs(w / n - x)) < 1e-14
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+def test_ordered_large_bit_exact(cfft, oracle, algo):
+    # cfft_ordered_plan_create_large: 2^11 <= n <= 2^13 against the oracle's
+    # ordered recursion at the same sizes (itself pinned to the closed-form
+    # DFT on the CPU)
+    from cfft_amd.ordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(40 + algo)
+    for n in (2048, 4096, 8192):
+        plan = Plan(n, Method.UserProvided(FftAlgo(algo)), large=True)
+        assert plan.fft_size() == n and plan.algo() == FftAlgo(algo)
+        x = rand_c(rng, (3, n))
+        y = x.copy()
+        plan.fwd(y)
+        assert np.array_equal(y, oracle.ordered_fwd(x, algo)), (algo, n)
+        w = y.copy()
+        plan.inv(w)
+        assert np.array_equal(w, oracle.ordered_inv(y, algo)), (algo, n)
+        assert np.max(np.abs(w / n - x)) < 1e-14
+    # below 2^11 the large entry point builds the same plan as the capped one
+    small = Plan(512, Method.UserProvided(FftAlgo(algo)), large=True)
+    x = rand_c(rng, (2, 512))
+    y = x.copy()
+    small.fwd(y)
+    assert np.array_equal(y, oracle.ordered_fwd(x, algo))
+
+
 @pytest.mark.parametrize("n", [32, 64, 256, 1024, 2048, 4096, 8192, 16384])
 def test_unordered_bit_exact(cfft, oracle, n):
     from cfft_amd.unordered import FftAlgo, Method, Plan

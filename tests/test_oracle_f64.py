@@ -85,6 +85,23 @@ def test_ordered_vs_numpy(O, algo):
         assert np.max(np.abs(z - x)) < 1e-14, n
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+def test_ordered_large_vs_numpy(O, algo):
+    # the ordered recursion beyond the reference's n < 2^11 assertion: no
+    # reference output exists, so the closed-form DFT (pocketfft) and the
+    # round trip pin it; the capped entry keeps refusing these sizes
+    rng = np.random.default_rng(200 + algo)
+    for n in (2048, 4096, 8192):
+        x = rng.random((2, n)) + 1j * rng.random((2, n))
+        y = O.ordered_fwd(x, algo)
+        ref = np.fft.fft(x, axis=-1)
+        assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 1e-14, n
+        z = O.ordered_inv(y, algo) / n
+        assert np.max(np.abs(z - x)) < 1e-14, n
+    buf = np.zeros(2048, np.complex128)
+    assert O.L.or_ordered_fwd_batch(2048, algo, buf.ctypes.data, 1) != 0
+
+
 @pytest.mark.parametrize("algo", [0, 3, 6])
 def test_ordered_vs_bruteforce_mpmath(O, algo):
     mpmath.mp.prec = 100
