@@ -94,6 +94,13 @@ class Plan {
     check(cfft_ordered_plan_create(n, m.kind, (int)m.algo, m.seconds, &p));
     h_ = detail::PlanHandle(p);
   }
+  // 2^11 <= n <= 2^13 as well: the reference's ordered recursion beyond its
+  // n < 2^11 assertion (cfft_ordered_plan_create_large)
+  static Plan large(size_t n, Method m) {
+    cfft_plan p = nullptr;
+    check(cfft_ordered_plan_create_large(n, m.kind, (int)m.algo, m.seconds, &p));
+    return Plan(detail::PlanHandle(p));
+  }
   size_t fft_size() const { return detail::fft_size(h_.get()); }
   FftAlgo algo() const {
     int a = 0;
@@ -123,6 +130,7 @@ class Plan {
   cfft_plan raw() const { return h_.get(); }
 
  private:
+  explicit Plan(detail::PlanHandle h) : h_(std::move(h)) {}
   detail::PlanHandle h_;
 };
 }  // namespace ordered

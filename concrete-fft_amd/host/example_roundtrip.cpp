@@ -30,6 +30,19 @@ int main() {
     std::printf("ordered n=4 roundtrip err %.3g\n", e);
     failures += e >= 1e-9;
   }
+  // an ordered plan beyond the reference's n < 2^11 assertion
+  {
+    const size_t n = 4096;
+    auto plan = ordered::Plan::large(n, ordered::Method::user_provided(FftAlgo::Dif16));
+    std::vector<c64> z(n);
+    for (auto &v : z) v = c64(u(rng), u(rng));
+    std::vector<c64> orig = z;
+    plan.fwd(z);
+    plan.inv(z);
+    const double e = max_err(z, orig, 1.0 / (double)n);
+    std::printf("ordered n=%zu (large) roundtrip err %.3g\n", n, e);
+    failures += e >= 1e-12;
+  }
   // test_roundtrip over unordered plans
   for (size_t n : {32, 64, 256, 512, 1024}) {
     unordered::Plan plan(n, unordered::Method::user_provided(FftAlgo::Dif4, 32));

@@ -18,3 +18,4 @@ def test_cpp_mirror_roundtrips(cfft):
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
     assert "panic ok: assertion failed: n.is_power_of_two()" in r.stdout
+    assert "ordered n=4096 (large) roundtrip" in r.stdout
