@@ -54,8 +54,8 @@ namespace cfft_internal {
 int set_error(int status, const std::string &msg);
 int hip_error(hipError_t e, const char *where);
 
-// Fourier scratch the split external product of `batch` GLWEs needs (0 when
-// the batch runs fused, one workgroup per GLWE).
+// Fourier scratch the split external product of `batch` GLWEs needs: the
+// spectra of one pass (large batches run in passes over the same scratch).
 size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int levels, size_t batch);
 // cfft_external_product with caller-owned scratch: `scratch` is used when it
 // holds external_product_scratch_bytes(...) bytes, else the scratch cache.

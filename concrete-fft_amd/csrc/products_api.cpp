@@ -42,8 +42,21 @@ hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint
 }
 
 namespace {
-// batch * k1 * levels at or below which the external product runs split
-constexpr uint64_t kSplitRowBlocks = 1024;
+// The external product runs split (rows and columns over workgroups through a
+// Fourier scratch) at every batch size: measured faster than one workgroup
+// per GLWE from batch 1 to 4096 at N = 1024, 2048 and 4096.  Large batches
+// run in passes over one scratch of at most kSplitMaxRows spectra and
+// kSplitMaxBytes, a whole number of GLWEs per pass.
+constexpr uint64_t kSplitMaxRows = 16384;
+constexpr uint64_t kSplitMaxBytes = (uint64_t)256 << 20;
+
+uint64_t glwe_per_pass(const cfft_plan_s *plan, int k1, int levels) {
+  const uint64_t per_glwe = (uint64_t)k1 * (uint64_t)levels;
+  uint64_t rows = kSplitMaxBytes / (plan->n * sizeof(cfft_launch::dc64));
+  if (rows > kSplitMaxRows) rows = kSplitMaxRows;
+  const uint64_t g = rows / per_glwe;
+  return g > 0 ? g : 1;
+}
 
 int check_extprod_plan(cfft_plan plan) {
   if (!plan) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
@@ -63,8 +76,9 @@ uint32_t ilog2(size_t n) {
 namespace cfft_internal {
 
 size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int levels, size_t batch) {
-  const uint64_t rows = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
-  return rows <= kSplitRowBlocks ? rows * plan->n * sizeof(cfft_launch::dc64) : 0;
+  const uint64_t per_pass = glwe_per_pass(plan, k1, levels);
+  const uint64_t m = (uint64_t)batch < per_pass ? (uint64_t)batch : per_pass;
+  return m * (uint64_t)k1 * (uint64_t)levels * plan->n * sizeof(cfft_launch::dc64);
 }
 
 int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k1, int levels,
@@ -87,52 +101,60 @@ int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k
                                       "not fit in one workgroup's LDS");
   if (batch == 0) return CFFT_SUCCESS;
   if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
-  // Small batches: one workgroup per GLWE leaves most of the device idle and
-  // runs k1 * levels + k1 transforms back to back; the split form spreads the
-  // rows and columns over workgroups (through a Fourier scratch) instead.
-  // Same accumulation order, same bits.
-  const uint64_t row_blocks = (uint64_t)batch * (uint64_t)k1 * (uint64_t)levels;
-  if (row_blocks <= kSplitRowBlocks) {
-    hipStream_t s = stream;
-    const size_t bytes = row_blocks * plan->n * sizeof(cfft_launch::dc64);
-    // the caller's scratch when it holds the split form's spectra, else the cache
-    cfft_internal::ScratchLease lease;
-    if (!scratch || scratch_bytes < bytes) {
-      e = cfft_internal::scratch_acquire(bytes, s, &lease);
-      if (e != hipSuccess) return hip_error(e, "external product scratch");
+  // One workgroup per GLWE (the fused kernel) leaves most of the device idle
+  // at small batches and runs k1 * levels + k1 transforms back to back; the
+  // split form spreads the rows and columns over workgroups instead.  Same
+  // accumulation order, same bits.
+  const uint64_t per_pass = glwe_per_pass(plan, k1, levels);
+  const size_t bytes = external_product_scratch_bytes(plan, k1, levels, batch);
+  hipStream_t s = stream;
+  // the caller's scratch when it holds one pass's spectra, else the cache
+  cfft_internal::ScratchLease lease;
+  if (!scratch || scratch_bytes < bytes) {
+    e = cfft_internal::scratch_acquire(bytes, s, &lease);
+    if (e == hipErrorOutOfMemory) {
+      // no room for the spectra: the fused kernel needs no scratch
+      (void)hipGetLastError();
+      return hip_error(cfft_launch::external_product(
+                           plan->fwd_full, plan->inv_full,
+                           reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out,
+                           batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
+                           plan->d_twist, plan->d_untwist, stream),
+                       "external product");
     }
-    void *S = lease.ptr ? lease.ptr : scratch;
+    if (e != hipSuccess) return hip_error(e, "external product scratch");
+  }
+  cfft_launch::dc64 *S = reinterpret_cast<cfft_launch::dc64 *>(lease.ptr ? lease.ptr : scratch);
+  const size_t glwe_words = (size_t)k1 * 2 * plan->n;  // one GLWE: k1 polynomials of N words
+  for (uint64_t off = 0; off < batch && e == hipSuccess; off += per_pass) {
+    const uint64_t m = batch - off < per_pass ? batch - off : per_pass;
+    const uint64_t rows = m * (uint64_t)k1 * (uint64_t)levels;
+    const uint64_t *g = glwe + off * glwe_words;
+    uint64_t *o = out + off * glwe_words;
     // Dif16 plans: one wave per row and per column (wave-local
     // register-resident transforms; two per wave at n = 512) at n = 512 and
-    // 1024; at n = 2048
-    // only for up to 8 rows (measured: from 8 GLWEs on, two waves per row
-    // transform in the generic split form give the lower latency)
+    // 1024; at n = 2048 only for up to 8 rows (measured: from 8 rows on, two
+    // waves per row transform in the generic split form give the lower
+    // latency)
     const cfft_launch::dc64 *blk =
         plan->n == 1024 && plan->d_tw_r16                       ? plan->d_tw_r16
         : plan->n == 512 && plan->d_tw_base1024                 ? plan->d_tw_base1024
-        : plan->n == 2048 && row_blocks <= 8 && plan->d_tw_base1024 ? plan->d_tw_base1024
-                                                                     : nullptr;
+        : plan->n == 2048 && rows <= 8 && plan->d_tw_base1024   ? plan->d_tw_base1024
+                                                                 : nullptr;
     if (blk)
       e = cfft_launch::external_product_split_w(
-          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out,
-          batch, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk,
-          plan->d_twist, plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
+          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), g, o, m,
+          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk, plan->d_twist,
+          plan->d_untwist, S, s);
     else
       e = cfft_launch::external_product_split(
-          plan->fwd_full, plan->inv_full,
-          reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
-          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, plan->d_twist,
-          plan->d_untwist, reinterpret_cast<cfft_launch::dc64 *>(S), s);
-    hipError_t e2 = cfft_internal::scratch_release(lease, s);
-    if (e != hipSuccess) return hip_error(e, "external product (split)");
-    return hip_error(e2, "external product scratch release");
+          plan->fwd_full, plan->inv_full, reinterpret_cast<const cfft_launch::dc64 *>(key_fourier),
+          g, o, m, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, plan->d_twist,
+          plan->d_untwist, S, s);
   }
-  return hip_error(cfft_launch::external_product(
-                       plan->fwd_full, plan->inv_full,
-                       reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), glwe, out, batch,
-                       (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw,
-                       plan->d_twist, plan->d_untwist, stream),
-                   "external product");
+  hipError_t e2 = cfft_internal::scratch_release(lease, s);
+  if (e != hipSuccess) return hip_error(e, "external product (split)");
+  return hip_error(e2, "external product scratch release");
 }
 
 }  // namespace cfft_internal

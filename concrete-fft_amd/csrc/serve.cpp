@@ -186,7 +186,7 @@ size_t req_out_bytes(int kind, const Key *k, uint32_t log2N) {
 }
 
 // device bytes beyond inputs and outputs: the split external product's
-// Fourier scratch for `m` GLWEs of one key (0 when they run fused)
+// Fourier scratch for `m` GLWEs of one key (one pass's spectra)
 size_t key_scratch(const Key &k, size_t m) {
   return round_up(cfft_internal::external_product_scratch_bytes(k.plan, k.k1, k.levels, m));
 }

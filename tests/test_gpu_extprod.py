@@ -104,21 +104,39 @@ def test_external_product_errors(cfft):
 
 
 @pytest.mark.parametrize("N", [1024, 2048, 4096])
-def test_fused_and_split_forms_agree(cfft, oracle, N):
-    # batch * k1 * levels above 1024 runs the fused one-workgroup-per-GLWE
-    # kernel; smaller calls run the split row / column kernels (one wave per
-    # row and column on the Dif16 plans at N = 2048 and 4096)
+def test_batch_splits_agree(cfft, oracle, N):
+    # one call over 200 GLWEs equals two calls over 100 each (different
+    # launch shapes of the split row / column kernels) and the oracle
     k1, levels, base_log = 2, 3, 8
     rng = np.random.default_rng(N + 99)
     key = rng.integers(0, M, (k1 * levels, k1, N), dtype=np.uint64)
     glwe = rng.integers(0, M, (200, k1, N), dtype=np.uint64)
     plan = _plan(N, 6, min(N // 2, 1024))
     kf = _key_fourier(plan, key, k1, levels)
-    fused = _extprod(plan, kf, k1, levels, base_log, glwe)
-    split = np.concatenate([_extprod(plan, kf, k1, levels, base_log, glwe[i:i + 100])
+    whole = _extprod(plan, kf, k1, levels, base_log, glwe)
+    parts = np.concatenate([_extprod(plan, kf, k1, levels, base_log, glwe[i:i + 100])
                             for i in (0, 100)])
-    assert np.array_equal(fused, split)
+    assert np.array_equal(whole, parts)
     kfh = oracle.torus_to_fourier(key, 6, min(N // 2, 1024))
     for b in (0, 199):
-        assert np.array_equal(fused[b], oracle.external_product(N, k1, levels, base_log, kfh, 6,
+        assert np.array_equal(whole[b], oracle.external_product(N, k1, levels, base_log, kfh, 6,
                                                                 min(N // 2, 1024), glwe[b]))
+
+
+def test_large_batch_runs_in_passes(cfft, oracle):
+    # k1 * levels = 16 rows per GLWE: 16384 rows make one pass of 1024 GLWEs,
+    # so 1100 GLWEs run in two passes over the same scratch
+    N, k1, levels, base_log = 1024, 2, 8, 4
+    rng = np.random.default_rng(321)
+    key = rng.integers(0, M, (k1 * levels, k1, N), dtype=np.uint64)
+    glwe = rng.integers(0, M, (1100, k1, N), dtype=np.uint64)
+    plan = _plan(N, 6, 512)
+    kf = _key_fourier(plan, key, k1, levels)
+    whole = _extprod(plan, kf, k1, levels, base_log, glwe)
+    parts = np.concatenate([_extprod(plan, kf, k1, levels, base_log, glwe[i:j])
+                            for i, j in ((0, 500), (500, 1100))])
+    assert np.array_equal(whole, parts)
+    kfh = oracle.torus_to_fourier(key, 6, 512)
+    for b in (0, 1023, 1024, 1099):
+        assert np.array_equal(whole[b], oracle.external_product(N, k1, levels, base_log, kfh, 6,
+                                                                512, glwe[b])), b
