@@ -55,6 +55,7 @@ def test_ggsw_to_fourier_bit_exact(cfft, oracle, N, algo, base_n):
     (512, 3, 4, 6, 7, 256, 7),
     (64, 2, 1, 16, 6, 32, 9),
     (4, 2, 3, 8, 0, 2, 3),
+    (4096, 2, 3, 8, 2, 1024, 40),  # generic split kernels (Dif4 plan), 240 rows
 ])
 def test_external_product_bit_exact(cfft, oracle, N, k1, levels, base_log, algo, base_n, batch):
     rng = np.random.default_rng(N * 7 + k1 + levels)
