@@ -443,11 +443,10 @@ hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, u
 hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
                        uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
+  // (products run on unordered plans, whose base_n is at most 1024)
   switch (log2n) {
     case 9: return launch_polymul<9>(a, b, c, batch, tw, twist, s);
     case 10: return launch_polymul<10>(a, b, c, batch, tw, twist, s);
-    case 11: return launch_polymul<11>(a, b, c, batch, tw, twist, s);
-    case 12: return launch_polymul<12>(a, b, c, batch, tw, twist, s);
     default: return hipErrorInvalidValue;
   }
 }
