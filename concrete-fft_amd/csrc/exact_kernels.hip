@@ -28,18 +28,14 @@ using cfft_prog::Program;
 
 namespace {
 
-// balanced 16-bit limb `li` of v (carry from the lower limbs; the carry out of
-// the top limb vanishes mod 2^64)
+// balanced 16-bit limb `li` of v: the digits d_i in [-2^15, 2^15) with
+// v = sum d_i 2^(16 i) mod 2^64 (the carry out of the top limb vanishes).
+// With e_i = d_i + 2^15 in [0, 2^16), sum e_i 2^(16 i) = v + 0x8000...8000
+// mod 2^64, so the e_i are the base-2^16 digits of that sum (the digits are
+// unique); the same as propagating carries limb by limb.
 __device__ __forceinline__ int64_t limb16(uint64_t v, uint32_t li) {
-  int64_t carry = 0, d = 0;
-#pragma unroll
-  for (uint32_t q = 0; q < 4; ++q) {
-    d = (int64_t)((v >> (16 * q)) & 0xffffu) + carry;
-    carry = d >= 32768 ? 1 : 0;
-    d -= carry << 16;
-    if (q == li) break;
-  }
-  return d;
+  const uint64_t e = v + 0x8000800080008000ull;
+  return (int64_t)((e >> (16 * li)) & 0xffffu) - 32768;
 }
 
 template <int E>
