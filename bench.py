@@ -301,8 +301,9 @@ def bench_exact(steps, warmup):
 
 
 def bench_transforms(steps=50):
-    """Batched unordered forward transforms in place on device, 64 Mi points
-    per call: n = 1024 (N = 2048) x 8192 and n = 2048 (N = 4096) x 4096;
+    """Batched unordered forward transforms in place on device, 8 Mi points
+    per call: n = 512 x 16384, n = 1024 (N = 2048) x 8192 and n = 2048
+    (N = 4096) x 4096;
     transforms/s and HBM fraction at 32 n bytes each."""
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -311,7 +312,8 @@ def bench_transforms(steps=50):
     s = Stream()
     x = DeviceBuffer.from_host(rng.uniform(-1, 1, (8192, 2048)).view(np.complex128), s)
     out = {}
-    for n, algo, base in ((1024, FftAlgo.Dif16, 1024), (1024, FftAlgo.Dif8, 1024),
+    for n, algo, base in ((512, FftAlgo.Dif16, 512),
+                          (1024, FftAlgo.Dif16, 1024), (1024, FftAlgo.Dif8, 1024),
                           (1024, FftAlgo.Dit16, 1024), (1024, FftAlgo.Dif16, 512),
                           (2048, FftAlgo.Dif16, 1024), (2048, FftAlgo.Dif8, 1024)):
         batch = 8192 * 1024 // n

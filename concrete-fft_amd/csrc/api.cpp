@@ -279,6 +279,9 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   }
   const uint32_t log2n = ilog2(p->n);
   dc64 *d = reinterpret_cast<dc64 *>(buf);
+  if (p->n == 512 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, wave-local
+    return hip_error(cfft_launch::w512_transform(fwd, d, batch, stride, p->d_tw_base1024, s),
+                     "w512_transform");
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
     return hip_error(cfft_launch::r16_transform(log2n, fwd, d, batch, stride, p->d_tw_r16, s),
                      "r16_transform");

@@ -358,9 +358,69 @@ k_w1024_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
+// ---- Dif16 plan with n = base_n = 512: 2 x NV transforms per wavefront ----
+// Each 32-lane half of a wave runs NV transforms (r16l_transform<9>, twiddles
+// read from the 1024-point block at column stride 2, twiddle reads shared),
+// 8 waves per workgroup sharing that block staged in LDS; each half has its
+// own 544-point exchange buffer.  A half whose transform would lie past the
+// batch repeats the last one and stores nothing.
+template <bool FWD, int NV>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_w512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
+                 const c64 *__restrict__ tw_blk) {
+  using S = R16<9>;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * (2 * NV);
+  if (first >= batch) return;
+  const uint32_t half = (threadIdx.x >> 5) & 1u, l = threadIdx.x & 31u;
+  const uint64_t t0 = first + half * NV;
+  const bool st0 = t0 < batch, st1 = NV == 2 && t0 + 1 < batch;
+  c64 *d0 = data + (st0 ? t0 : batch - 1) * stride;
+  c64 *d1 = st1 ? d0 + stride : d0;
+  c64 *x = smem + kTwWords + (wave * 2 + half) * S::lds_words;
+  c64 va[16], vb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    va[m] = ld_stream(&d0[l + 32 * m]);
+    if (NV == 2) vb[m] = ld_stream(&d1[l + 32 * m]);
+  }
+  r16l_transform<9, FWD, NV>(va, vb, x, l, ltw);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t q = l + 32 * S::end_m(s);
+    if (st0) st_stream(va[s], &d0[q]);
+    if (st1) st_stream(vb[s], &d1[q]);
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
+
+hipError_t w512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                          const dc64 *tw_blk, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  constexpr int NV = 2;
+  const void *k = fwd ? reinterpret_cast<const void *>(k_w512_transform<true, NV>)
+                      : reinterpret_cast<const void *>(k_w512_transform<false, NV>);
+  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  if (e != hipSuccess) return e;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw_blk);
+  const uint64_t per_wg = (uint64_t)kPW * 2 * NV;
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
+  if (fwd)
+    hipLaunchKernelGGL((k_w512_transform<true, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+                       batch, stride, t);
+  else
+    hipLaunchKernelGGL((k_w512_transform<false, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+                       batch, stride, t);
+  return hipGetLastError();
+}
 
 hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                            const dc64 *tw_r16, hipStream_t s) {

@@ -75,7 +75,7 @@ def test_ordered_large_bit_exact(cfft, oracle, algo):
     assert np.array_equal(y, oracle.ordered_fwd(x, algo))
 
 
-@pytest.mark.parametrize("n", [32, 64, 256, 1024, 2048, 4096, 8192, 16384])
+@pytest.mark.parametrize("n", [32, 64, 256, 512, 1024, 2048, 4096, 8192, 16384])
 def test_unordered_bit_exact(cfft, oracle, n):
     from cfft_amd.unordered import FftAlgo, Method, Plan
     rng = np.random.default_rng(n)
@@ -111,13 +111,15 @@ def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
         assert np.max(np.abs(w / n - x)) < 1e-12
 
 
-@pytest.mark.parametrize("n,stride,batch", [(1024, 1100, 7), (2048, 2050, 11)])
+@pytest.mark.parametrize("n,stride,batch", [(512, 530, 9), (512, 512, 33), (1024, 1100, 7),
+                                            (2048, 2050, 11)])
 def test_device_path_with_stride(cfft, oracle, n, stride, batch):
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
     rng = np.random.default_rng(1)
     x = rand_c(rng, (batch, stride))
-    plan = Plan(n, Method.UserProvided(FftAlgo.Dif16, 1024))
+    base = min(n, 1024)
+    plan = Plan(n, Method.UserProvided(FftAlgo.Dif16, base))
     s = Stream()
     d = DeviceBuffer.from_host(x)
     plan.fwd(d, stream=s, batch=batch)
@@ -127,13 +129,13 @@ def test_device_path_with_stride(cfft, oracle, n, stride, batch):
     cfft_amd.check(cfft_amd.lib.cfft_fwd(plan.handle, d2.dptr, batch, stride, s.handle))
     s.synchronize()
     y = d2.download()
-    ref = oracle.unordered_fwd(np.ascontiguousarray(x[:, :n]), 6, 1024)
+    ref = oracle.unordered_fwd(np.ascontiguousarray(x[:, :n]), 6, base)
     assert np.array_equal(y[:, :n], ref)
     assert np.array_equal(y[:, n:], x[:, n:]), "padding between transforms was touched"
     cfft_amd.check(cfft_amd.lib.cfft_inv(plan.handle, d2.dptr, batch, stride, s.handle))
     s.synchronize()
     w = d2.download()
-    assert np.array_equal(w[:, :n], oracle.unordered_inv(ref, 6, 1024))
+    assert np.array_equal(w[:, :n], oracle.unordered_inv(ref, 6, base))
     assert np.array_equal(w[:, n:], x[:, n:])
 
 
