@@ -55,6 +55,11 @@ hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint
 hipError_t w512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                           const dc64 *tw_blk, hipStream_t s);
 
+// Negacyclic product (N = 1024) on a Dif16 plan with n = base_n = 512: one
+// product per 32-lane half-wave.
+hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                        const dc64 *tw_blk, const dc64 *twist, hipStream_t s);
+
 // Dif16 plans with n = base_n = 1024: two transforms per wavefront
 // (r16w_transform), 8 wavefronts per workgroup; tw_r16 is the plan's r16
 // twiddle block.

@@ -101,6 +101,19 @@ def test_polymul_small_outputs_bit_exact(cfft, oracle):
     assert not c[0].any()
 
 
+@pytest.mark.parametrize("batch", [1, 2, 15, 16, 17, 1001])
+def test_half_wave_products_n1024(cfft, oracle, batch):
+    # N = 1024 on a Dif16 plan with base 512: one product per 32-lane
+    # half-wave, 16 per workgroup; odd batches leave the last half idle
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 1024
+    a, b = rand_inputs(batch + 5, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 512))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 512))
+
+
 @pytest.mark.parametrize("batch", [1, 7, 8, 9, 2049])
 def test_persistent_grid_boundaries(cfft, oracle, batch):
     # N = 2048 Dif16 / 1024 runs the persistent kernel: 8 products per
