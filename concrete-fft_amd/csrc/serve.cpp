@@ -43,6 +43,9 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr size_t kAlign = 256;
+// an idle dispatcher polls for new submissions this long before it blocks on
+// the condition variable (a blocked thread's wake-up adds tens of us)
+constexpr int kIdleSpinUs = 500;
 constexpr int kMaxKeys = 256;
 constexpr int kMaxLog2N = 14;
 
@@ -163,6 +166,7 @@ struct cfft_server_s {
   std::deque<Req> waiting;
   // statistics
   std::atomic<uint64_t> n_batches{0}, n_done{0}, n_admission_waits{0}, max_inflight{0};
+  std::atomic<uint64_t> n_submitted{0};  // bumped after each push to `pending`
 
   ~cfft_server_s();
   void run();
@@ -397,6 +401,11 @@ void cfft_server_s::run() {
     }
     // 3. wait
     if (inflight.empty() && waiting.empty()) {
+      // a bounded spin catches the next submission without a wake-up
+      const uint64_t seen = n_submitted.load(std::memory_order_acquire);
+      const Clock::time_point spin_end = Clock::now() + std::chrono::microseconds(kIdleSpinUs);
+      while (n_submitted.load(std::memory_order_acquire) == seen && Clock::now() < spin_end)
+        __builtin_ia32_pause();
       std::unique_lock<std::mutex> g(mu);
       cv_submit.wait(g, [&] { return stop || !pending.empty(); });
     } else {
@@ -548,6 +557,7 @@ int cfft_server_submit(cfft_server s, const cfft_request *r, uint64_t *ticket) {
     s->tickets[q.ticket] = TicketState{};
     s->pending.push_back(q);
   }
+  s->n_submitted.fetch_add(1, std::memory_order_release);
   s->cv_submit.notify_one();
   *ticket = q.ticket;
   return CFFT_SUCCESS;
