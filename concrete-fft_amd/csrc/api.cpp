@@ -222,7 +222,10 @@ int build_f64_plan(cfft_plan_s *p) {
                          hipMemcpyHostToDevice));
     }
   }
-  if (p->algo == CFFT_DIF16 && p->base_n == 1024 && n > 1024 && n <= 8192) {
+  // the 1024-point r16 block of the plan's table, for the wave-local forms
+  // with a radix-2 depth pass (n = 2 base_n) and the uneven / product kernels
+  if (p->algo == CFFT_DIF16 && ((p->base_n == 1024 && n > 1024 && n <= 8192) ||
+                                (p->base_n == 512 && n == 1024))) {
     const size_t mul = n / 1024;
     std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);
     for (size_t k = 0; k < 16; ++k) {
@@ -288,6 +291,10 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   if (p->d_tw_r8)
     return hip_error(cfft_launch::r8_transform(fwd, d, batch, stride, p->d_tw_r8, s),
                      "r8_transform");
+  if (p->n == 1024 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, base_n = 512
+    return hip_error(cfft_launch::wd2_512_transform(fwd, d, batch, stride, p->d_tw,
+                                                    p->d_tw_base1024, s),
+                     "wd2_512_transform");
   if (p->n == 2048 && p->d_tw_base1024)  // Dif16, base_n = 1024
     return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),

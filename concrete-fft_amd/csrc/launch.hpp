@@ -60,6 +60,12 @@ hipError_t w512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
 hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                         const dc64 *tw_blk, const dc64 *twist, hipStream_t s);
 
+// Dif16 plans with n = 1024 and base_n = 512: one transform per 32-lane
+// half-wave (r16l_depth2_transform<9>); tw is the plan's table, tw_blk its
+// 1024-point r16 block.
+hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                             const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
+
 // Dif16 plans with n = base_n = 1024: two transforms per wavefront
 // (r16w_transform), 8 wavefronts per workgroup; tw_r16 is the plan's r16
 // twiddle block.

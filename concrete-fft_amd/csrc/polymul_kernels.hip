@@ -452,9 +452,70 @@ k_w512_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   }
 }
 
+// ---- Dif16 plan with n = 1024 and base_n = 512: one transform per half-wave ----
+// r16l_depth2_transform<9> (the radix-2 depth pass in registers around the
+// two 512-point base transforms, run together), two transforms per wave,
+// 8 waves per workgroup sharing the 1024-point block in LDS; each half has
+// its own 544-point exchange buffer.  A half past the batch repeats the last
+// transform and stores nothing.
+template <bool FWD>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_wd2_512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
+                    const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk) {
+  using S = R16<9>;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * 2;
+  if (first >= batch) return;
+  const uint32_t half = (threadIdx.x >> 5) & 1u, l = threadIdx.x & 31u;
+  const uint64_t t = first + half;
+  const bool act = t < batch;
+  c64 *d = data + (act ? t : batch - 1) * stride;
+  c64 *x = smem + kTwWords + (wave * 2 + half) * S::lds_words;
+  c64 va[16], vb[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) {
+    va[m] = ld_stream(&d[l + 32 * m]);
+    vb[m] = ld_stream(&d[512 + l + 32 * m]);
+  }
+  r16l_depth2_transform<9, FWD>(va, vb, x, l, ltw, tw);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const uint32_t q = l + 32 * S::end_m(s);
+    if (act) {
+      st_stream(va[s], &d[q]);
+      st_stream(vb[s], &d[512 + q]);
+    }
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
+
+hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                             const dc64 *tw, const dc64 *tw_blk, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  const void *k = fwd ? reinterpret_cast<const void *>(k_wd2_512_transform<true>)
+                      : reinterpret_cast<const void *>(k_wd2_512_transform<false>);
+  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  if (e != hipSuccess) return e;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
+  const uint64_t per_wg = (uint64_t)kPW * 2;
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
+  if (fwd)
+    hipLaunchKernelGGL(k_wd2_512_transform<true>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+                       stride, t, tb);
+  else
+    hipLaunchKernelGGL(k_wd2_512_transform<false>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+                       stride, t, tb);
+  return hipGetLastError();
+}
 
 hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                         const dc64 *tw_blk, const dc64 *twist, hipStream_t s) {

@@ -367,24 +367,28 @@ __device__ __forceinline__ void r16l_transform(c64 *va, c64 *vb, c64 *x, uint32_
   }
 }
 
-// A Dif16 plan with n = 2048 and base_n = 1024 on one wavefront.  The
-// plan is one radix-2 depth pass (src/unordered.rs:391-489) around two
-// 1024-point base transforms; the depth butterfly pairs the points j and
-// 1024 + j, which lane l holds in the same slot (va[m] = point l + 64 m,
-// vb[m] = point 1024 + l + 64 m), so the depth pass runs in registers and
-// the two base transforms run together (NV = 2, twiddle reads shared).
-//   forward: in  va[m], vb[m] as above; out va[s] = point l + 64 end_m(s),
-//            vb[s] = point 1024 + l + 64 end_m(s)   (plan order)
-//   inverse: in  va[m] = point l + 64 m, vb[m] = point 1024 + l + 64 m (plan
-//            order); out va[s] = point q, vb[s] = point 1024 + q with
-//            q = l + 64 end_m(s)
-// `tw` is the plan's full table T[j] (the depth twiddles T[j], j < 1024, in
-// global memory), `twl` the 1024-point r16 block in LDS, `x` the wave's
-// exchange buffer (R16<10>::lds_words points).
-template <bool FWD>
-__device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
-                                                    const c64 *twl, const c64 *__restrict__ tw) {
-  using S = R16<10>;
+// A Dif16 plan with n = 2 b and base_n = b (b = 2^LOG2B = 512 or 1024) on
+// the T = b / 16 lanes of one transform group (a half-wave at b = 512, a
+// whole wave at b = 1024).  The plan is one radix-2 depth pass
+// (src/unordered.rs:391-489) around two b-point base transforms; the depth
+// butterfly pairs the points j and b + j, which lane l holds in the same slot
+// (va[m] = point l + T m, vb[m] = point b + l + T m), so the depth pass runs
+// in registers and the two base transforms run together (NV = 2, twiddle
+// reads shared).
+//   forward: in  va[m], vb[m] as above; out va[s] = point l + T end_m(s),
+//            vb[s] = point b + l + T end_m(s)   (plan order)
+//   inverse: in  va[m] = point l + T m, vb[m] = point b + l + T m (plan
+//            order); out va[s] = point q, vb[s] = point b + q with
+//            q = l + T end_m(s)
+// `tw` is the plan's full table T[j] (the depth twiddles T[j], j < b, in
+// global memory), `twl` the 1024-point r16 block in LDS (read at column
+// stride 1024 / b), `x` the group's exchange buffer (R16<LOG2B>::lds_words
+// points).
+template <int LOG2B, bool FWD>
+__device__ __forceinline__ void r16l_depth2_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                                      const c64 *twl,
+                                                      const c64 *__restrict__ tw) {
+  using S = R16<LOG2B>;
   // a forward and an inverse of one job read the same depth twiddles; merged
   // loads would stay live across a whole transform
   tw = launder(tw);
@@ -394,7 +398,7 @@ __device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, ui
     for (uint32_t m0 = 0; m0 < 16; m0 += 4) {
       c64 w[4];
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e) w[e] = tw[l + 64 * (m0 + e)];
+      for (uint32_t e = 0; e < 4; ++e) w[e] = tw[l + S::T * (m0 + e)];
 #pragma unroll
       for (uint32_t e = 0; e < 4; ++e) {
         c64 t2[2] = {va[m0 + e], vb[m0 + e]};
@@ -404,16 +408,16 @@ __device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, ui
       }
       sched_fence();
     }
-    r16l_transform<10, true, 2>(va, vb, x, l, twl);
+    r16l_transform<LOG2B, true, 2>(va, vb, x, l, twl);
   } else {
-    r16l_transform<10, false, 2>(va, vb, x, l, twl);
+    r16l_transform<LOG2B, false, 2>(va, vb, x, l, twl);
     // DEPTH_INV radix 2: conj(T[q]) on chunk 1's point before the butterfly
 #pragma unroll
     for (uint32_t s0 = 0; s0 < 16; s0 += 4) {
       c64 w[4];
 #pragma unroll
       for (uint32_t e = 0; e < 4; ++e) {
-        w[e] = tw[l + 64 * S::end_m(s0 + e)];
+        w[e] = tw[l + S::T * S::end_m(s0 + e)];
         w[e].im = -w[e].im;
       }
 #pragma unroll
@@ -426,6 +430,13 @@ __device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, ui
       sched_fence();
     }
   }
+}
+
+// n = 2048, base_n = 1024 on one wavefront (T = 64)
+template <bool FWD>
+__device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                                    const c64 *twl, const c64 *__restrict__ tw) {
+  r16l_depth2_transform<10, FWD>(va, vb, x, l, twl, tw);
 }
 
 }  // namespace cfft_dev

@@ -111,14 +111,14 @@ def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
         assert np.max(np.abs(w / n - x)) < 1e-12
 
 
-@pytest.mark.parametrize("n,stride,batch", [(512, 530, 9), (512, 512, 33), (1024, 1100, 7),
-                                            (2048, 2050, 11)])
-def test_device_path_with_stride(cfft, oracle, n, stride, batch):
+@pytest.mark.parametrize("n,stride,batch,base", [(512, 530, 9, 512), (512, 512, 33, 512),
+                                                 (1024, 1100, 7, 1024), (1024, 1030, 7, 512),
+                                                 (1024, 1024, 32, 512), (2048, 2050, 11, 1024)])
+def test_device_path_with_stride(cfft, oracle, n, stride, batch, base):
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
     rng = np.random.default_rng(1)
     x = rand_c(rng, (batch, stride))
-    base = min(n, 1024)
     plan = Plan(n, Method.UserProvided(FftAlgo.Dif16, base))
     s = Stream()
     d = DeviceBuffer.from_host(x)
