@@ -415,7 +415,7 @@ def main():
         print(json.dumps({"mode": "serve", "serve": bench_serve(args.serve_seconds)}))
         return 0
     if args.mode == "exact":
-        print(json.dumps({"mode": "exact", "exact": bench_exact(10, 2)}))
+        print(json.dumps({"mode": "exact", "exact": bench_exact(50, 10)}))
         return 0
     if args.mode == "ordered":
         print(json.dumps({"mode": "ordered", "ordered": bench_ordered()}))
@@ -489,7 +489,7 @@ def main():
     if args.mode == "all":
         out["uneven"] = bench_uneven(args.uneven_steps, 2)
         out["serve"] = bench_serve(args.serve_seconds)
-        out["exact"] = bench_exact(10, 2)
+        out["exact"] = bench_exact(50, 10)
         out["ordered"] = bench_ordered()
         out["transforms"] = bench_transforms()
 
