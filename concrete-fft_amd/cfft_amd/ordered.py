@@ -108,7 +108,8 @@ class _PlanBase:
 
 
 class Plan(_PlanBase):
-    """ordered::Plan (src/ordered.rs:186-373): standard order in and out, n <= 1024."""
+    """ordered::Plan (src/ordered.rs:186-373): standard order in and out,
+    n <= 1024 (with large=True, n <= 8192)."""
 
     def __init__(self, n, method, large=False):
         """large=True admits 2^11 <= n <= 2^13 (cfft_ordered_plan_create_large):

@@ -10,8 +10,9 @@ namespace {
 
 constexpr size_t kMaxEntries = 8;
 constexpr size_t kMaxBytes = (size_t)2 << 30;
-// 32 MiB holds any split-form external product (at most 1024 rows of 2048
-// points) and any small exact product
+// 32 MiB holds the split-form external product of up to 1024 spectra of 2048
+// points (serving-size batches) and the small exact products; larger calls
+// take larger power-of-two entries
 constexpr size_t kMinEntry = (size_t)32 << 20;
 
 // entry sizes are powers of two (at least kMinEntry), so the few entries made
