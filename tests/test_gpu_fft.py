@@ -75,6 +75,19 @@ def test_ordered_large_bit_exact(cfft, oracle, algo):
     assert np.array_equal(y, oracle.ordered_fwd(x, algo))
 
 
+def test_ordered_large_measure(cfft, oracle):
+    # the device autotuner over the 8 algorithms at an extended size: the
+    # chosen plan computes that algorithm's bits
+    from cfft_amd.ordered import Method, Plan
+    rng = np.random.default_rng(77)
+    plan = Plan(4096, Method.Measure(0.002), large=True)
+    algo = int(plan.algo())
+    x = rand_c(rng, (2, 4096))
+    y = x.copy()
+    plan.fwd(y)
+    assert np.array_equal(y, oracle.ordered_fwd(x, algo)), algo
+
+
 @pytest.mark.parametrize("n", [32, 64, 256, 512, 1024, 2048, 4096, 8192, 16384])
 def test_unordered_bit_exact(cfft, oracle, n):
     from cfft_amd.unordered import FftAlgo, Method, Plan
