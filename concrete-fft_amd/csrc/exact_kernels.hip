@@ -232,7 +232,7 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
   double worst = 0.0;
   c64 v[16];
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) v[m] = wk[l + S::T * m];
+  for (uint32_t m = 0; m < 16; ++m) v[m] = ld_stream(&wk[l + S::T * m]);
   r16w_transform<false, 1>(v, v, x, l, ltw);
   uint64_t part[32];
   const c64 *tws = launder(twist);
@@ -390,8 +390,8 @@ k_exact_inv_w2(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *marg
 #pragma unroll
   for (uint32_t m = 0; m < 16; ++m) {
     const uint32_t q = l + 64 * m;
-    va[m] = wk[q];
-    vb[m] = wk[1024 + q];
+    va[m] = ld_stream(&wk[q]);
+    vb[m] = ld_stream(&wk[1024 + q]);
   }
   r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
   // the transform's last LDS reads are behind its final wave fence; the
