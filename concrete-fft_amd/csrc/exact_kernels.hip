@@ -201,7 +201,7 @@ k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
           c64 d = cmul(wa[0], wb[k]);
 #pragma unroll
           for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(wa[i], wb[k - i]));
-          W[k * n + q] = d;
+          st_stream(d, &W[k * n + q]);
         }
       }
     }
@@ -362,7 +362,7 @@ k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
         c64 d = cmul(fa[0], fb[k]);
 #pragma unroll
         for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(fa[i], fb[k - i]));
-        W[k * n + q] = d;
+        st_stream(d, &W[k * n + q]);
       }
     }
     __syncthreads();  // the buffers are rewritten for the second half
