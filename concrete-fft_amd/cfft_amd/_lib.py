@@ -122,6 +122,7 @@ _SIGS = [
     ("cfft_server_submit", c_int, [c_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]),
     ("cfft_server_poll", c_int, [c_vp, c_u64, ctypes.POINTER(c_int)]),
     ("cfft_server_wait", c_int, [c_vp, c_u64]),
+    ("cfft_server_release", c_int, [c_vp, c_u64]),
     ("cfft_server_replay", c_int, [c_vp, ctypes.POINTER(Request), c_size, c_vp, c_vp]),
     ("cfft_server_stats", c_int, [c_vp, ctypes.POINTER(ServerStatistics)]),
     ("cfft_uneven_info", c_int, [c_vp, ctypes.POINTER(c_size), ctypes.POINTER(c_size),

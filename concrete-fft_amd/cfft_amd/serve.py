@@ -22,6 +22,7 @@ class Server:
         check(lib.cfft_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h.value
         self._live = {}
+        self._released = []
         self._keys = {}
 
     def add_key(self, key, k1, levels, base_log):
@@ -57,11 +58,25 @@ class Server:
         return t.value
 
     def wait(self, ticket):
-        try:
-            check(lib.cfft_server_wait(self.handle, ticket))
-        finally:
-            keep = self._live.pop(ticket, None)
+        keep = self._live.get(ticket)
+        st = lib.cfft_server_wait(self.handle, ticket)
+        # an unknown or already-waited ticket (status 10) leaves the buffers
+        # to the thread that owns the wait
+        if st != 10:
+            self._live.pop(ticket, None)
+        check(st)
         return keep[-1]
+
+    def poll(self, ticket):
+        done = ctypes.c_int()
+        check(lib.cfft_server_poll(self.handle, ticket, ctypes.byref(done)))
+        return bool(done.value)
+
+    def release(self, ticket):
+        """Gives up a ticket without waiting; the request's output is still
+        written, so its buffers stay referenced until the server is closed."""
+        check(lib.cfft_server_release(self.handle, ticket))
+        self._released.append(self._live.pop(ticket, None))
 
     def replay(self, requests, arrival_s):
         """requests: list of Request structs (their buffers kept alive by the

@@ -764,7 +764,7 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
   if (p->n == 512 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, wave-local
     return hip_error(cfft_launch::w512_polymul(a, b, c, batch, p->d_tw_base1024, p->d_twist, s),
                      "w512_polymul");
-  if (p->algo == CFFT_DIF16 && p->base_n == p->n && cfft_launch::r16_supported(log2n))
+  if (p->algo == CFFT_DIF16 && p->base_n == p->n && p->n == 1024 && p->d_tw_r16)
     return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw_r16, p->d_twist, s),
                      "r16_polymul");
   if (p->d_tw_r8)

@@ -264,22 +264,23 @@ __device__ __forceinline__ double torus_to_f64(uint64_t a) {
 // Nearest multiple of 2^-64 of frac(x), as a u64.  Reference formula (half
 // away from zero in both roundings):
 //   fract = x - round(x); v = round(fract * 2^64); (u64)(i64)v, v = 2^63 -> 2^63.
-// Branch-free form with the same bits for every finite x:
-//   t = x - rint(x) is exact; it differs from fract only at half-integers,
-//   where both are +-1/2 and map to 2^63 (t = +1/2 is folded to -1/2 so that
-//   T = t 2^64 stays below 2^63);  T = t 2^64 is exact;  round half away
-//   from zero (a no-op once |T| >= 2^52);  then the integer T, |T| <= 2^63,
-//   is split exactly into floor(T / 2^32) and the remainder and converted
-//   to 32-bit halves.
+// Branch-free form with the same bits for every finite x (11 VALU operations):
+//   t = x - rint(x) is exact and differs from fract only at half-integers,
+//   where both are +-1/2, i.e. the same torus element 2^63;  T = t 2^64 is
+//   exact, |T| <= 2^63;  trunc(T + copysign(0.5 - 2^-54, T)) rounds half away
+//   from zero exactly (the sum is correctly rounded, and a value below a
+//   half-integer can never round up to it);  the integer Tr, |Tr| <= 2^63,
+//   splits exactly into hi = floor(Tr / 2^32) in [-2^31, 2^31] and
+//   lo = Tr - hi 2^32 in [0, 2^32);  hi + 1.5 2^52 holds hi mod 2^32 (two's
+//   complement, hi = 2^31 included) in its low mantissa word.
 __device__ __forceinline__ uint64_t from_torus(double x) {
-  double t = x - __builtin_rint(x);
-  t = t == 0.5 ? -0.5 : t;
-  const double T = t * 18446744073709551616.0;
-  const double Tt = __builtin_trunc(T);
-  const double Tr = fabs(T - Tt) >= 0.5 ? Tt + __builtin_copysign(1.0, T) : Tt;
-  const double hi = __builtin_floor(Tr * 2.3283064365386963e-10);
+  const double t = x - __builtin_rint(x);
+  const double T = t * 18446744073709551616.0;                                  // 2^64
+  const double Tr = __builtin_trunc(T + __builtin_copysign(0.49999999999999994, T));
+  const double hi = __builtin_floor(Tr * 2.3283064365386963e-10);               // 2^-32
   const double lo = __fma_rn(hi, -4294967296.0, Tr);
-  return ((uint64_t)(uint32_t)(int32_t)hi << 32) | (uint64_t)(uint32_t)lo;
+  const double hm = hi + 6755399441055744.0;                                    // 1.5 2^52
+  return ((uint64_t)(uint32_t)__builtin_bit_cast(uint64_t, hm) << 32) | (uint64_t)(uint32_t)lo;
 }
 
 }  // namespace cfft_dev

@@ -71,67 +71,10 @@ __global__ void __launch_bounds__(512) k_torus_to_fourier(const uint64_t *__rest
   run_program<E>(fwd, lds, lt, T, tw, load, store);
 }
 
-// One GLWE per transform group.  LDS: work buffer + k1 accumulators.
-template <int E>
-__global__ void __launch_bounds__(512) k_extprod(const c64 *__restrict__ keyf,
-                                                 const uint64_t *__restrict__ glwe,
-                                                 uint64_t *__restrict__ out, uint64_t batch,
-                                                 uint32_t k1, uint32_t levels, uint32_t base_log,
-                                                 const c64 *__restrict__ tw,
-                                                 const c64 *__restrict__ twist,
-                                                 const c64 *__restrict__ untwist, Program fwd,
-                                                 Program inv) {
-  extern __shared__ c64 smem[];
-  const uint32_t n = 1u << fwd.log2n;
-  const uint32_t T = n / E;
-  const uint32_t G = blockDim.x / T;
-  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
-  const uint64_t p = (uint64_t)blockIdx.x * G + g;
-  const bool active = p < batch;
-  const uint32_t words = lds_words(n);
-  c64 *lds = smem + g * (1 + k1) * words;
-  c64 *acc = lds + words;
-  for (uint32_t i = lt; i < k1 * words; i += T) acc[i] = mk(0.0, 0.0);
-  __syncthreads();
-  const uint64_t *in = glwe + p * (uint64_t)k1 * 2ull * n;
-  const uint32_t rows = k1 * levels;
-  for (uint32_t r = 0; r < rows; ++r) {
-    const uint32_t poly = r / levels, lvl = r % levels;
-    const uint64_t *src = in + (uint64_t)poly * 2ull * n;
-    auto load = [&](uint32_t j) -> c64 {
-      if (!active) return mk(0.0, 0.0);
-      return cmul(twist[j],
-                  mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
-                     (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
-    };
-    const c64 *krow = keyf + (uint64_t)r * k1 * n;
-    auto store = [&](uint32_t pos, c64 v) {
-      for (uint32_t c = 0; c < k1; ++c) {
-        c64 &a = acc[c * words + lds_pad(pos)];
-        const c64 pr = cmul(krow[(uint64_t)c * n + pos], v);
-        a = mk(a.re + pr.re, a.im + pr.im);
-      }
-    };
-    run_program<E>(fwd, lds, lt, T, tw, load, store);
-  }
-  for (uint32_t c = 0; c < k1; ++c) {
-    const c64 *ac = acc + c * words;
-    auto load = [&](uint32_t pos) -> c64 { return ac[lds_pad(pos)]; };
-    uint64_t *o = out + (p * k1 + c) * 2ull * n;
-    auto store = [&](uint32_t j, c64 v) {
-      if (!active) return;
-      const c64 r = cmul(untwist[j], v);
-      st_stream(from_torus(r.re), &o[j]);
-      st_stream(from_torus(r.im), &o[j + n]);
-    };
-    run_program<E>(inv, lds, lt, T, tw, load, store);
-  }
-}
-
-// Split form for small batches (latency): one workgroup per (GLWE, row)
-// transforms its digit polynomial into a Fourier scratch S[(p rows + r) n];
-// then one workgroup per (GLWE, column) accumulates sum_r key[r][c] * S[p, r]
-// in the same row order (same bits as the fused kernel), inverse, torus.
+// Split form: one transform group per (GLWE, row) transforms its digit
+// polynomial into a Fourier scratch S[(p rows + r) n]; then one per (GLWE,
+// column) accumulates sum_r key[r][c] * S[p, r] in ascending row order,
+// inverse, torus.
 template <int E>
 __global__ void __launch_bounds__(512) k_extprod_rows(const uint64_t *__restrict__ glwe,
                                                       c64 *__restrict__ S, uint64_t nblocks,
@@ -457,23 +400,6 @@ hipError_t launch_key(const Program &fwd, const uint64_t *key, c64 *out, uint64_
 }
 
 template <int E>
-hipError_t launch_ext(const Program &fwd, const Program &inv, const c64 *keyf,
-                      const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
-                      uint32_t levels, uint32_t base_log, const c64 *tw, const c64 *twist,
-                      const c64 *untwist, hipStream_t s) {
-  const uint32_t words = lds_words(1u << fwd.log2n);
-  const uint32_t gw = (1 + k1) * words;
-  const Geom g = geom(fwd.log2n, 80 * 1024, gw);
-  const size_t lds = (size_t)g.G * gw * sizeof(c64);
-  hipError_t e = set_lds(reinterpret_cast<const void *>(k_extprod<E>), lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_extprod<E>, dim3((uint32_t)((batch + g.G - 1) / g.G)), dim3(g.threads),
-                     lds, s, keyf, glwe, out, batch, k1, levels, base_log, tw, twist, untwist,
-                     fwd, inv);
-  return hipGetLastError();
-}
-
-template <int E>
 hipError_t launch_split(const Program &fwd, const Program &inv, const c64 *keyf,
                         const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
                         uint32_t levels, uint32_t base_log, const c64 *tw, const c64 *twist,
@@ -504,10 +430,6 @@ int pick_e(uint32_t log2n) {
 
 namespace cfft_launch {
 
-size_t external_product_lds_bytes(uint32_t log2n, uint32_t k1) {
-  return (size_t)(1 + k1) * lds_words(1u << log2n) * sizeof(c64);
-}
-
 hipError_t torus_to_fourier(const Program &fwd, const uint64_t *key, dc64 *out, uint64_t npolys,
                             const dc64 *tw, const dc64 *twist, hipStream_t s) {
   if (npolys == 0) return hipSuccess;
@@ -518,23 +440,6 @@ hipError_t torus_to_fourier(const Program &fwd, const uint64_t *key, dc64 *out, 
     case 4: return launch_key<4>(fwd, key, o, npolys, t, z, s);
     case 8: return launch_key<8>(fwd, key, o, npolys, t, z, s);
     default: return launch_key<16>(fwd, key, o, npolys, t, z, s);
-  }
-}
-
-hipError_t external_product(const Program &fwd, const Program &inv, const dc64 *keyf,
-                            const uint64_t *glwe, uint64_t *out, uint64_t batch, uint32_t k1,
-                            uint32_t levels, uint32_t base_log, const dc64 *tw,
-                            const dc64 *twist, const dc64 *untwist, hipStream_t s) {
-  if (batch == 0) return hipSuccess;
-  const c64 *kf = reinterpret_cast<const c64 *>(keyf);
-  const c64 *t = reinterpret_cast<const c64 *>(tw), *z = reinterpret_cast<const c64 *>(twist);
-  const c64 *u = reinterpret_cast<const c64 *>(untwist);
-  switch (pick_e(fwd.log2n)) {
-    case 2: return launch_ext<2>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
-    case 4: return launch_ext<4>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
-    case 8: return launch_ext<8>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
-    default:
-      return launch_ext<16>(fwd, inv, kf, glwe, out, batch, k1, levels, base_log, t, z, u, s);
   }
 }
 

@@ -41,57 +41,40 @@ __global__ void __launch_bounds__(R16<LOG2N>::T) k_r16_transform(c64 *__restrict
   for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &x[l + S::T * S::end_m(s)]);
 }
 
-template <int LOG2N>
-__global__ void __launch_bounds__(R16<LOG2N>::T) __attribute__((amdgpu_waves_per_eu(2, 2))) k_r16_polymul(
-    const uint64_t *__restrict__ A, const int64_t *__restrict__ B, uint64_t *__restrict__ C,
-    uint64_t batch, const c64 *__restrict__ tw, const c64 *__restrict__ twist) {
-  using S = R16<LOG2N>;
-  constexpr uint32_t n = S::n;
-  __shared__ c64 lds[S::lds_words];
-  const uint32_t l = threadIdx.x;
-  const double inv_n = 1.0 / (double)n;
-  {
-    const uint64_t poly = blockIdx.x;
-    if (poly >= batch) return;
-    const c64 *twist_i = twist;
-    const uint64_t *a = A + poly * (2 * n);
-    const int64_t *b = B + poly * (2 * n);
-    uint64_t *c = C + poly * (2 * n);
-    c64 fa[16], v[16];
-    // A, B and the twist are fetched together: one memory latency per product
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) {
-      const uint32_t j = l + S::T * m;
-      const c64 z = twist_i[j];
-      fa[m] = cmul(z, mk(torus_to_f64(ld_stream(&a[j])), torus_to_f64(ld_stream(&a[j + n]))));
-      v[m] = cmul(z, mk(i64_to_f64(ld_stream(&b[j])), i64_to_f64(ld_stream(&b[j + n]))));
-    }
-    r16_transform<LOG2N, true>(fa, lds, l, tw);
-    sched_fence();
-    r16_transform<LOG2N, true>(v, lds, l, tw);
-    sched_fence();
-    c64 z[16];
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
-    r16_end_to_input<LOG2N>(fa, z);
-    r16_transform<LOG2N, false>(z, lds, l, tw);
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-      const uint32_t j = l + S::T * S::end_m(s);
-      const c64 r = cmul(untwist_from(twist_i[j], inv_n), z[s]);
-      st_stream(from_torus(r.re), &c[j]);
-      st_stream(from_torus(r.im), &c[j + n]);
-      if ((s & 3) == 3) sched_fence();
-    }
-  }
-}
-
 // Persistent product kernel for n = 1024 (N = 2048): 8 waves per workgroup,
-// each running its own stream of products; the r16 twiddle block is staged
-// once per workgroup in LDS and shared by the waves.
+// each running its own stream of products.  LDS holds the r16 twiddle rows
+// k = 1..15 (1020 points), the twist table (1024) and each wave's unpadded
+// 1024-point exchange buffer: 163 776 bytes, one workgroup per CU.
+//
+// The kernel runs at 2 waves per SIMD (up to 256 VGPRs: a product's two
+// spectra are live together) and, with HBM and the FP64 units both busy, at
+// the device's power limit (shader clock measured 1.9-2.1 GHz against
+// 2.4 GHz for its loads alone or its arithmetic alone), so work per product
+// is what its time follows:
+//  * twist and untwist factors come from LDS: no global load is issued after
+//    the prefetch below, so its latency is never waited for early (vector
+//    memory counters complete in order);
+//  * the 1/n of the inverse transform is folded into A's conversion
+//    (power-of-two scalings are exact, the untwist is then conj(zeta_j)
+//    itself, the same bits);
+//  * B of the wave's next product is loaded during the inverse transform;
+//    past its last product a wave reads the L2-resident twist table instead
+//    (unconditional loads keep the registers unspilled; no HBM traffic).
 constexpr uint32_t kPW = 8;  // waves per workgroup
 constexpr uint32_t kTwWords = 16 * 64 + 16 * 4;
-constexpr size_t kPersistLds = (kTwWords + kPW * R16<10>::lds_words) * sizeof(c64);
+constexpr uint32_t kPmTw1 = 15 * 64, kPmTw2 = 15 * 4, kPmTwist = 1024, kPmX = 1024;
+constexpr uint32_t kPmHead = kPmTw1 + kPmTw2 + kPmTwist;
+constexpr size_t kPersistLds = (size_t)(kPmHead + kPW * kPmX) * sizeof(c64);
+// the wave-local kernels below keep the padded exchange and the whole block
+constexpr size_t kWaveLds = (kTwWords + kPW * R16<10>::lds_words) * sizeof(c64);
+
+// (double)(int64)a * 2^-64 / 1024 with one rounding (2^-74 and 2^-42 scale
+// exactly)
+__device__ __forceinline__ double torus_to_f64_over_n1024(uint64_t a) {
+  const double hi = (double)(int32_t)(a >> 32);
+  const double lo = (double)(uint32_t)a * 0x1p-74;
+  return __fma_rn(hi, 0x1p-42, lo);
+}
 
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
@@ -100,56 +83,71 @@ k_r16_polymul_persistent(const uint64_t *__restrict__ A, const int64_t *__restri
   using S = R16<10>;
   constexpr uint32_t n = S::n;
   extern __shared__ c64 smem[];
-  c64 *ltw = smem;
-  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_r16[i];
+  c64 *ltw1 = smem, *ltw2 = smem + kPmTw1, *ltz = smem + kPmTw1 + kPmTw2;
+  for (uint32_t i = threadIdx.x; i < kPmTw1; i += blockDim.x) ltw1[i] = tw_r16[64 + i];
+  for (uint32_t i = threadIdx.x; i < kPmTw2; i += blockDim.x) ltw2[i] = tw_r16[1024 + 4 + i];
+  for (uint32_t i = threadIdx.x; i < kPmTwist; i += blockDim.x) ltz[i] = twist[i];
   __syncthreads();
   // uniform per wave: keeps product indices and base addresses in SGPRs
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  c64 *x = smem + kTwWords + wave * S::lds_words;
-  const double inv_n = 1.0 / (double)n;
-  for (uint64_t poly = (uint64_t)blockIdx.x * kPW + wave; poly < batch;
-       poly += (uint64_t)gridDim.x * kPW) {
+  c64 *x = smem + kPmHead + wave * kPmX;
+  const uint64_t step = (uint64_t)gridDim.x * kPW;
+  uint64_t poly = (uint64_t)blockIdx.x * kPW + wave;
+  uint64_t pb[32];  // raw words of B for the wave's current product
+  if (poly < batch) {
+    const uint32_t l = threadIdx.x & 63u;
+    const int64_t *b = B + poly * (2 * n);
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      pb[2 * m] = (uint64_t)ld_stream(&b[l + S::T * m]);
+      pb[2 * m + 1] = (uint64_t)ld_stream(&b[l + S::T * m + n]);
+    }
+  }
+  for (; poly < batch; poly += step) {
     // recomputed per product so per-lane addresses are not kept live across
     // the loop
     const uint32_t l = launder_u32(threadIdx.x) & 63u;
     const uint64_t *a = A + poly * (2 * n);
-    const int64_t *b = B + poly * (2 * n);
     uint64_t *c = C + poly * (2 * n);
-    const c64 *tws = launder(twist);
     c64 fa[16], v[16];
-    // every load of the product (A, B, twist: 192 VGPRs, nothing else is live
-    // here) is issued before any arithmetic, so the product waits for one
-    // memory latency, not for several register-limited rounds of loads
     {
       uint64_t ra[32];
-      int64_t rb[32];
-      c64 tz[16];
 #pragma unroll
       for (uint32_t m = 0; m < 16; ++m) {
         const uint32_t j = l + S::T * m;
         ra[2 * m] = ld_stream(&a[j]);
         ra[2 * m + 1] = ld_stream(&a[j + n]);
-        rb[2 * m] = ld_stream(&b[j]);
-        rb[2 * m + 1] = ld_stream(&b[j + n]);
-        tz[m] = tws[j];
       }
       sched_fence();
 #pragma unroll
       for (uint32_t m = 0; m < 16; ++m) {
-        fa[m] = cmul(tz[m], mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
-        v[m] = cmul(tz[m], mk(i64_to_f64(rb[2 * m]), i64_to_f64(rb[2 * m + 1])));
+        const c64 z = ltz[l + S::T * m];
+        fa[m] = cmul(z, mk(torus_to_f64_over_n1024(ra[2 * m]),
+                           torus_to_f64_over_n1024(ra[2 * m + 1])));
+        v[m] = cmul(z, mk(i64_to_f64((int64_t)pb[2 * m]), i64_to_f64((int64_t)pb[2 * m + 1])));
       }
     }
-    r16w_transform<true, 2>(fa, v, x, l, ltw);
+    r16x_transform<true, 2>(fa, v, x, l, ltw1, ltw2);
     c64 z[16];
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
     r16_end_to_input<10>(fa, z);
-    r16w_transform<false, 1>(z, z, x, l, ltw);
+    sched_fence();
+    {
+      const uint64_t nx = poly + step;
+      const int64_t *bn = nx < batch ? B + nx * (2 * n) : reinterpret_cast<const int64_t *>(twist);
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        pb[2 * m] = (uint64_t)ld_stream(&bn[l + S::T * m]);
+        pb[2 * m + 1] = (uint64_t)ld_stream(&bn[l + S::T * m + n]);
+      }
+    }
+    sched_fence();
+    r16x_transform<false, 1>(z, z, x, l, ltw1, ltw2);
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) {
       const uint32_t j = l + S::T * S::end_m(s);
-      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
+      const c64 r = cmul(conj(ltz[j]), z[s]);
       st_stream(from_torus(r.re), &c[j]);
       st_stream(from_torus(r.im), &c[j + n]);
       if ((s & 3) == 3) sched_fence();
@@ -201,19 +199,6 @@ hipError_t launch_polymul_persistent(const uint64_t *a, const int64_t *b, uint64
   const uint32_t grid = (uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus);
   hipLaunchKernelGGL(k_r16_polymul_persistent, dim3(grid), dim3(64 * kPW), kPersistLds, s, a, b,
                      c, batch, reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
-  return hipGetLastError();
-}
-
-template <int LOG2N>
-hipError_t launch_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
-                          const dc64 *tw, const dc64 *twist, hipStream_t s) {
-  using S = R16<LOG2N>;
-  // n = 1024: one product per wave, 8 waves per workgroup sharing the
-  // twiddles staged in LDS (measured faster than one workgroup per product)
-  if (LOG2N == 10) return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
-  const uint32_t grid = (uint32_t)batch;
-  hipLaunchKernelGGL(k_r16_polymul<LOG2N>, dim3(grid), dim3(S::T), 0, s, a, b, c, batch,
-                     reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(twist));
   return hipGetLastError();
 }
 
@@ -501,7 +486,7 @@ hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stri
   if (batch == 0) return hipSuccess;
   const void *k = fwd ? reinterpret_cast<const void *>(k_wd2_512_transform<true>)
                       : reinterpret_cast<const void *>(k_wd2_512_transform<false>);
-  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw);
@@ -509,10 +494,10 @@ hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stri
   const uint64_t per_wg = (uint64_t)kPW * 2;
   const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
   if (fwd)
-    hipLaunchKernelGGL(k_wd2_512_transform<true>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+    hipLaunchKernelGGL(k_wd2_512_transform<true>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
   else
-    hipLaunchKernelGGL(k_wd2_512_transform<false>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+    hipLaunchKernelGGL(k_wd2_512_transform<false>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
   return hipGetLastError();
 }
@@ -521,11 +506,11 @@ hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64
                         const dc64 *tw_blk, const dc64 *twist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_w512_polymul),
-                                         kPersistLds);
+                                         kWaveLds);
   if (e != hipSuccess) return e;
   const uint64_t per_wg = (uint64_t)kPW * 2;
   hipLaunchKernelGGL(k_w512_polymul, dim3((uint32_t)((batch + per_wg - 1) / per_wg)),
-                     dim3(64 * kPW), kPersistLds, s, a, b, c, batch,
+                     dim3(64 * kPW), kWaveLds, s, a, b, c, batch,
                      reinterpret_cast<const c64 *>(tw_blk), reinterpret_cast<const c64 *>(twist));
   return hipGetLastError();
 }
@@ -536,17 +521,17 @@ hipError_t w512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
   constexpr int NV = 2;
   const void *k = fwd ? reinterpret_cast<const void *>(k_w512_transform<true, NV>)
                       : reinterpret_cast<const void *>(k_w512_transform<false, NV>);
-  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw_blk);
   const uint64_t per_wg = (uint64_t)kPW * 2 * NV;
   const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
   if (fwd)
-    hipLaunchKernelGGL((k_w512_transform<true, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+    hipLaunchKernelGGL((k_w512_transform<true, NV>), grid, dim3(64 * kPW), kWaveLds, s, d,
                        batch, stride, t);
   else
-    hipLaunchKernelGGL((k_w512_transform<false, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+    hipLaunchKernelGGL((k_w512_transform<false, NV>), grid, dim3(64 * kPW), kWaveLds, s, d,
                        batch, stride, t);
   return hipGetLastError();
 }
@@ -557,17 +542,17 @@ hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
   constexpr int NV = 2;
   const void *k = fwd ? reinterpret_cast<const void *>(k_w1024_transform<true, NV>)
                       : reinterpret_cast<const void *>(k_w1024_transform<false, NV>);
-  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw_r16);
   const uint64_t per_wg = (uint64_t)kPW * NV;
   const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg));
   if (fwd)
-    hipLaunchKernelGGL((k_w1024_transform<true, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+    hipLaunchKernelGGL((k_w1024_transform<true, NV>), grid, dim3(64 * kPW), kWaveLds, s, d,
                        batch, stride, t);
   else
-    hipLaunchKernelGGL((k_w1024_transform<false, NV>), grid, dim3(64 * kPW), kPersistLds, s, d,
+    hipLaunchKernelGGL((k_w1024_transform<false, NV>), grid, dim3(64 * kPW), kWaveLds, s, d,
                        batch, stride, t);
   return hipGetLastError();
 }
@@ -577,17 +562,17 @@ hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
   if (batch == 0) return hipSuccess;
   const void *k = fwd ? reinterpret_cast<const void *>(k_w2048_transform<true>)
                       : reinterpret_cast<const void *>(k_w2048_transform<false>);
-  hipError_t e = cfft_launch::ensure_lds(k, kPersistLds);
+  hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw);
   const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
   const dim3 grid((uint32_t)((batch + kPW - 1) / kPW));
   if (fwd)
-    hipLaunchKernelGGL(k_w2048_transform<true>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+    hipLaunchKernelGGL(k_w2048_transform<true>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
   else
-    hipLaunchKernelGGL(k_w2048_transform<false>, grid, dim3(64 * kPW), kPersistLds, s, d, batch,
+    hipLaunchKernelGGL(k_w2048_transform<false>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
   return hipGetLastError();
 }
@@ -632,12 +617,10 @@ hipError_t r16_transform(uint32_t log2n, bool fwd, dc64 *data, uint64_t batch, u
 hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
                        uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  // (products run on unordered plans, whose base_n is at most 1024)
-  switch (log2n) {
-    case 9: return launch_polymul<9>(a, b, c, batch, tw, twist, s);
-    case 10: return launch_polymul<10>(a, b, c, batch, tw, twist, s);
-    default: return hipErrorInvalidValue;
-  }
+  // n = 1024: one product per wave, 8 waves per workgroup sharing the
+  // twiddles staged in LDS (n = 512 runs k_w512_polymul)
+  if (log2n != 10) return hipErrorInvalidValue;
+  return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
 }
 
 }  // namespace cfft_launch

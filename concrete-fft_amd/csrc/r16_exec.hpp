@@ -440,3 +440,76 @@ __device__ __forceinline__ void r16l_2048_transform(c64 *va, c64 *vb, c64 *x, ui
 }
 
 }  // namespace cfft_dev
+
+namespace cfft_dev {
+
+// ---- wave-local 1024-point transform with an unpadded exchange ----
+//
+// Same operations and layouts as r16w_transform; two differences free LDS
+// for the callers that keep more tables there:
+//  * the twiddle rows for k = 0 (T[0] = 1, never multiplied) are not stored:
+//    tw1[(k-1) 64 + l] = T[k l] and tw2[(k-1) 4 + p] = T[16 k p], k = 1..15;
+//  * the exchange buffer holds exactly 1024 points: point i is stored at
+//    i ^ ((i >> 4) & 15) (the low four index bits XORed with the next four),
+//    which spreads both the row writes (i = 16 l + k) and the column reads
+//    (i = l + 64 k) of 16 consecutive lanes over 16 distinct 16-byte bank
+//    groups, as a one-point pad per 16 would.
+template <bool FWD>
+__device__ __forceinline__ void twiddle_rows_x2(c64 *va, c64 *vb, int nv, const c64 *rows,
+                                                uint32_t row_stride, uint32_t col) {
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    c64 w[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) w[e] = tw_at_impl<FWD>(rows, (uint32_t)(k - 1) * row_stride + col);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = 4 * g + e;
+      if (k > 0) {
+        va[k] = cmul(w[e], va[k]);
+        if (nv == 2) vb[k] = cmul(w[e], vb[k]);
+      }
+    }
+    sched_fence();
+  }
+}
+
+template <bool FWD, int NV>
+__device__ __forceinline__ void r16x_transform(c64 *va, c64 *vb, c64 *x, uint32_t l,
+                                               const c64 *tw1, const c64 *tw2) {
+  using S = R16<10>;
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
+  sched_fence();
+  twiddle_rows_x2<FWD>(va, vb, NV, tw1, 64, l);
+  // exchange 1 (write 16 l + k, read l + 64 k; swizzled as above)
+  const uint32_t p = l >> 4, q = l & 15;
+#pragma unroll
+  for (int t = 0; t < NV; ++t) {
+    c64 *v = t == 0 ? va : vb;
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[16 * l + ((uint32_t)k ^ q)] = v[k];
+    wave_lds_fence();
+#pragma unroll
+    for (int k = 0; k < 16; ++k)
+      v[k] = x[64 * (uint32_t)k + 16 * p + (q ^ (p + 4 * ((uint32_t)k & 3)))];
+  }
+  wave_lds_fence();  // last use of x
+  butterfly<16, FWD>(va);
+  if (NV == 2) butterfly<16, FWD>(vb);
+  sched_fence();
+  twiddle_rows_x2<FWD>(va, vb, NV, tw2, 4, p);
+  r16_exchange2_rows<10>(va);
+  if (NV == 2) r16_exchange2_rows<10>(vb);
+#pragma unroll
+  for (uint32_t i = 0; i < S::NB; ++i) {
+    butterfly<S::REND, FWD>(va + i * S::REND);
+    if (NV == 2) butterfly<S::REND, FWD>(vb + i * S::REND);
+  }
+}
+
+}  // namespace cfft_dev

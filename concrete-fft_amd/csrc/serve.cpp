@@ -127,6 +127,8 @@ struct Batch {
 
 struct TicketState {
   bool done = false;
+  bool claimed = false;   // a thread is waiting on it (at most one may)
+  bool released = false;  // the caller gave it up: forgotten when it completes
   int status = CFFT_SUCCESS;
   Clock::time_point t_done;
 };
@@ -294,10 +296,15 @@ void cfft_server_s::retire(Batch &b, hipError_t err) {
   {
     std::lock_guard<std::mutex> g(mu);
     for (const Req &r : b.reqs) {
-      TicketState &t = tickets[r.ticket];
-      t.done = true;
-      t.status = err == hipSuccess ? CFFT_SUCCESS : CFFT_ERROR_DEVICE;
-      t.t_done = now;
+      auto it = tickets.find(r.ticket);
+      if (it == tickets.end()) continue;
+      if (it->second.released) {
+        tickets.erase(it);
+        continue;
+      }
+      it->second.done = true;
+      it->second.status = err == hipSuccess ? CFFT_SUCCESS : CFFT_ERROR_DEVICE;
+      it->second.t_done = now;
     }
   }
   n_done += b.reqs.size();
@@ -476,7 +483,7 @@ int cfft_server_add_key(cfft_server s, size_t N, int k1, int levels, int base_lo
   int rc = cfft_unordered_plan_create(n, CFFT_METHOD_USER_PROVIDED, CFFT_DIF16, n < 1024 ? n : 1024,
                                       0.0, &k.plan);
   if (rc) return rc;
-  // the shape must fit the external-product kernel (LDS accumulators)
+  // the GGSW shape must be one the external product accepts
   rc = cfft_external_product(k.plan, nullptr, k1, levels, base_log, nullptr, nullptr, 0, nullptr);
   if (rc) {
     cfft_plan_destroy(k.plan);
@@ -567,8 +574,22 @@ int cfft_server_poll(cfft_server s, uint64_t ticket, int *done) {
   if (!s || !done) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
   std::lock_guard<std::mutex> g(s->mu);
   auto it = s->tickets.find(ticket);
-  if (it == s->tickets.end()) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown ticket");
+  if (it == s->tickets.end() || it->second.released)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown ticket");
   *done = it->second.done ? 1 : 0;
+  return CFFT_SUCCESS;
+}
+
+int cfft_server_release(cfft_server s, uint64_t ticket) {
+  if (!s) return set_error(CFFT_ERROR_NULL_POINTER, "server is null");
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->tickets.find(ticket);
+  if (it == s->tickets.end() || it->second.released)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown ticket");
+  if (it->second.claimed)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "ticket is being waited on");
+  if (it->second.done) s->tickets.erase(it);
+  else it->second.released = true;
   return CFFT_SUCCESS;
 }
 
@@ -578,8 +599,15 @@ namespace {
 int wait_ticket(cfft_server s, uint64_t ticket, Clock::time_point *t_done) {
   std::unique_lock<std::mutex> g(s->mu);
   auto it = s->tickets.find(ticket);
-  if (it == s->tickets.end()) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown ticket");
-  s->cv_done.wait(g, [&] { return it->second.done; });
+  if (it == s->tickets.end() || it->second.released)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown ticket");
+  if (it->second.claimed)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "ticket is already being waited on");
+  it->second.claimed = true;
+  // submissions may rehash the map while this thread sleeps: look the entry
+  // up again after every wake-up (the claim keeps it from being erased)
+  s->cv_done.wait(g, [&] { return s->tickets.find(ticket)->second.done; });
+  it = s->tickets.find(ticket);
   const int st = it->second.status;
   if (t_done) *t_done = it->second.t_done;
   s->tickets.erase(it);

@@ -354,8 +354,14 @@ int cfft_server_add_key(cfft_server s, size_t N, int k1, int levels, int base_lo
 /* Non-blocking; CFFT_ERROR_BUSY if the request alone exceeds the budget. */
 int cfft_server_submit(cfft_server s, const cfft_request *request, uint64_t *ticket);
 int cfft_server_poll(cfft_server s, uint64_t ticket, int *done);
-/* Blocks until the request is done and its output written; forgets the ticket. */
+/* Blocks until the request is done and its output written; forgets the ticket.
+ * One thread at a time may wait on a ticket: a second waiter gets
+ * CFFT_ERROR_INVALID_ARGUMENT. */
 int cfft_server_wait(cfft_server s, uint64_t ticket);
+/* Gives up a ticket without waiting (for callers that poll): forgotten now if
+ * its request is done, else as soon as it completes; its output is still
+ * written.  A ticket somebody waits on cannot be released. */
+int cfft_server_release(cfft_server s, uint64_t ticket);
 /* Open-loop replay: submits request i at arrival_s[i] seconds after the start
  * (from this thread), waits for all, and stores each latency from its
  * scheduled arrival to its completion in latency_s[i]. */

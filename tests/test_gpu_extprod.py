@@ -96,8 +96,10 @@ def test_external_product_errors(cfft):
         products.external_product(plan, kf, 2, 3, 0, g, o, 1)
     with pytest.raises(CfftPanic, match="base_log"):
         products.external_product(plan, kf, 2, 9, 8, g, o, 1)
-    with pytest.raises(CfftPanic, match="LDS"):
-        products.external_product(plan, kf, 16, 1, 8, g, o, 1)
+    with pytest.raises(CfftPanic, match="k1"):
+        products.external_product(plan, kf, 65, 1, 8, g, o, 1)
+    with pytest.raises(CfftPanic, match="k1"):
+        products.external_product(plan, kf, 0, 3, 8, g, o, 1)
     from cfft_amd.ordered import FftAlgo, Method, Plan
     with pytest.raises(CfftPanic, match="unordered"):
         products.external_product(Plan(1024, Method.UserProvided(FftAlgo.Dif4)), kf, 2, 3, 8, g,

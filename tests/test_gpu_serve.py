@@ -178,3 +178,78 @@ def test_concurrent_submitters(cfft):
     for i in range(len(reqs)):
         assert np.array_equal(out[i], alone[i]), i
     srv.close()
+
+
+def test_ticket_lifetime(cfft):
+    """A polled ticket can be released; a released or waited ticket is
+    unknown afterwards; a ticket can be waited on once."""
+    import threading
+    import time
+
+    from cfft_amd import CfftPanic
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(7)
+    srv = Server()
+    a = rng.integers(0, M, 1024, dtype=np.uint64)
+    b = rng.integers(-128, 128, 1024, dtype=np.int64)
+    q, keep = srv.polymul_request(a, b)
+    t1 = srv.submit(q, keep)
+    deadline = time.time() + 10
+    while not srv.poll(t1):
+        assert time.time() < deadline
+        time.sleep(1e-4)
+    srv.release(t1)
+    with pytest.raises(CfftPanic):
+        srv.poll(t1)
+    with pytest.raises(CfftPanic):
+        srv.wait(t1)
+    # released before completion: forgotten when it completes, output written
+    q2, keep2 = srv.polymul_request(a, b)
+    t2 = srv.submit(q2, keep2)
+    srv.release(t2)
+    q3, keep3 = srv.polymul_request(a, b)
+    out3 = srv.wait(srv.submit(q3, keep3))
+    np.testing.assert_array_equal(out3, keep[-1])
+    # many tickets in flight while one thread waits (the map rehashes under it)
+    q4, keep4 = srv.polymul_request(a, b)
+    t4 = srv.submit(q4, keep4)
+    got = {}
+    th = threading.Thread(target=lambda: got.setdefault("r", srv.wait(t4)))
+    th.start()
+    more = [srv.submit(*srv.polymul_request(a, b)) for _ in range(300)]
+    th.join(30)
+    assert not th.is_alive()
+    np.testing.assert_array_equal(got["r"], keep[-1])
+    for t in more:
+        srv.wait(t)
+    srv.close()
+
+
+def test_two_waiters_one_ticket(cfft):
+    """Two threads waiting on one ticket: exactly one gets the result, the
+    other an error (never a read of a forgotten ticket)."""
+    import threading
+
+    from cfft_amd import CfftPanic
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(8)
+    srv = Server()
+    a = rng.integers(0, M, 2048, dtype=np.uint64)
+    b = rng.integers(-128, 128, 2048, dtype=np.int64)
+    for _ in range(20):
+        q, keep = srv.polymul_request(a, b)
+        t = srv.submit(q, keep)
+        res = []
+
+        def waiter():
+            try:
+                res.append(("ok", srv.wait(t)))
+            except CfftPanic as e:
+                res.append(("err", str(e)))
+        th = [threading.Thread(target=waiter) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join(30)
+        assert sorted(r[0] for r in res) == ["err", "ok"]
+    srv.close()

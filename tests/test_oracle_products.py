@@ -141,3 +141,41 @@ def test_external_product_coarse_grid_matches_schoolbook(O):
             want[c] += O.schoolbook_u64(key[r, c][None], digits[r].view(np.uint64)[None])[0]
     got = ((out + np.uint64(1 << 43)) >> np.uint64(44)) << np.uint64(44)
     assert np.array_equal(got, want)
+
+
+def _device_from_torus(x):
+    """The device kernels' branch-free torus rounding (device_math.hpp
+    from_torus), restated in numpy: every step is exact or a single
+    correctly rounded IEEE operation, as on the GPU."""
+    x = np.asarray(x, dtype=np.float64)
+    t = x - np.rint(x)
+    T = t * 2.0 ** 64
+    Tr = np.trunc(T + np.copysign(0.49999999999999994, T))
+    hi = np.floor(Tr * 2.0 ** -32)
+    lo = Tr - hi * 2.0 ** 32  # exact: the fma of the device form
+    hm = (hi + 6755399441055744.0).view(np.uint64) & np.uint64(0xFFFFFFFF)
+    return (hm << np.uint64(32)) | lo.astype(np.uint64)
+
+
+def test_device_torus_rounding_matches_oracle(O):
+    """The device's rounding formula gives the oracle's bits (the reference
+    rounding: fract = x - round(x), round(fract 2^64), 2^63 wrapped) on
+    random values over the whole range products produce and on the ties:
+    half-integers, odd multiples of 2^-65, neighbours of 2^-64 multiples and
+    +-1/2."""
+    rng = np.random.default_rng(11)
+    e = rng.integers(-100, 80, 200000)
+    xs = [np.ldexp(rng.random(200000), e) * rng.choice([-1.0, 1.0], 200000)]
+    k = np.arange(-300, 301, dtype=np.float64)
+    for base in (0.0, 0.25, 0.5, 0.75):
+        xs.append(k + base)
+    for ex in range(-100, 10, 3):
+        q = np.arange(-20, 21, dtype=np.float64)
+        xs.append(np.ldexp(q + 0.5, ex))
+        xs.append(np.ldexp(q, ex))
+    x = np.concatenate(xs)
+    x = np.concatenate([x, np.nextafter(x, np.inf), np.nextafter(x, -np.inf)])
+    got = _device_from_torus(x)
+    want = np.array([O.L.or_from_torus(float(v)) for v in x], dtype=np.uint64)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, (x[bad[:5]], got[bad[:5]], want[bad[:5]])
