@@ -125,3 +125,20 @@ def test_persistent_grid_boundaries(cfft, oracle, batch):
     plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
     c = products.negacyclic_polymul_u64_host(plan, a, b)
     assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 1024))
+
+
+@pytest.mark.parametrize("batch", [1, 8, 9, 2047, 2048, 2049, 8192, 8193])
+def test_persistent_product_batches(cfft, oracle, batch):
+    """N = 2048 on the Dif16 / 1024 plan (the persistent kernel: 8 waves per
+    workgroup, one workgroup per CU, products strided over waves, the next
+    product's B loaded during the inverse): bit-exact against the oracle at
+    batch sizes around its wave and round boundaries, waves without a
+    product included."""
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    a, b = rand_inputs(1000 + batch, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, 6, 1024, threads=16)
+    assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
