@@ -105,6 +105,9 @@ _SIGS = [
     ("cfft128_exact_polymul_u64", c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_size, c_vp]),
     ("cfft_ggsw_to_fourier", c_int, [c_vp, c_vp, c_int, c_int, c_vp, c_vp]),
     ("cfft_external_product", c_int, [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_size, c_vp]),
+    ("cfft_external_product_noalloc", c_int,
+     [c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_size, c_vp]),
+    ("cfft_noalloc_scratch_bytes", c_size, []),
     ("cfft_uneven_create", c_int, [c_vp, c_size, c_size, c_size, c_size, c_int, c_int, c_dbl,
                                    ctypes.POINTER(c_vp)]),
     ("cfft_uneven_destroy", c_int, [c_vp]),

@@ -79,3 +79,11 @@ def ggsw_to_fourier(plan, key, k1, levels, key_fourier, stream=None):
 def external_product(plan, key_fourier, k1, levels, base_log, glwe, out, batch, stream=None):
     check(lib.cfft_external_product(plan.handle, key_fourier.dptr, k1, levels, base_log,
                                     glwe.dptr, out.dptr, batch, _h(stream)))
+
+
+def external_product_noalloc(plan, key_fourier, k1, levels, base_log, glwe, out, batch,
+                             stream=None):
+    """external_product over the library's fixed per-device scratch: no device
+    memory is allocated (calls overlapping on one device race)."""
+    check(lib.cfft_external_product_noalloc(plan.handle, key_fourier.dptr, k1, levels, base_log,
+                                            glwe.dptr, out.dptr, batch, _h(stream)))

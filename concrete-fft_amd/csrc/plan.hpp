@@ -59,8 +59,10 @@ int hip_error(hipError_t e, const char *where);
 size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int levels, size_t batch);
 // cfft_external_product with caller-owned scratch: `scratch` is used when it
 // holds external_product_scratch_bytes(...) bytes, else the scratch cache.
+// With no_alloc the batch runs in passes of as many GLWEs as `scratch` holds
+// and nothing is allocated (CFFT_ERROR_SIZE if one GLWE does not fit).
 int external_product_with_scratch(cfft_plan_s *plan, const void *key_fourier, int k1, int levels,
                                   int base_log, const uint64_t *glwe, uint64_t *out,
                                   size_t batch, hipStream_t stream, void *scratch,
-                                  size_t scratch_bytes);
+                                  size_t scratch_bytes, bool no_alloc);
 }  // namespace cfft_internal

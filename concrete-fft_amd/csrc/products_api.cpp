@@ -3,6 +3,8 @@
 // resident Fourier-domain GGSW key.
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "cfft.h"
 #include "launch.hpp"
 #include "plan.hpp"
@@ -80,7 +82,7 @@ size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int level
 int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k1, int levels,
                                   int base_log, const uint64_t *glwe, uint64_t *out,
                                   size_t batch, hipStream_t stream, void *scratch,
-                                  size_t scratch_bytes) {
+                                  size_t scratch_bytes, bool no_alloc) {
   int st = check_extprod_plan(plan);
   if (st != CFFT_SUCCESS) return st;
   if (k1 < 1 || k1 > kMaxK1 || levels < 1)
@@ -94,12 +96,23 @@ int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k
   // Fourier scratch: measured faster than one workgroup per GLWE from batch 1
   // to 4096 at N = 1024, 2048 and 4096.
   hipError_t e = hipSuccess;
-  const uint64_t per_pass = glwe_per_pass(plan, k1, levels);
-  const size_t bytes = external_product_scratch_bytes(plan, k1, levels, batch);
+  uint64_t per_pass = glwe_per_pass(plan, k1, levels);
+  size_t bytes = external_product_scratch_bytes(plan, k1, levels, batch);
+  if (no_alloc) {
+    // passes sized to the caller's scratch; nothing is allocated
+    const uint64_t per_glwe = (uint64_t)k1 * (uint64_t)levels * plan->n * sizeof(cfft_launch::dc64);
+    if (!scratch || scratch_bytes < per_glwe)
+      return set_error(CFFT_ERROR_SIZE,
+                       "external product: one GLWE's spectra (" + std::to_string(per_glwe) +
+                           " bytes) do not fit the " + std::to_string(scratch_bytes) +
+                           "-byte scratch");
+    if (per_pass > scratch_bytes / per_glwe) per_pass = scratch_bytes / per_glwe;
+    bytes = 0;
+  }
   hipStream_t s = stream;
   // the caller's scratch when it holds one pass's spectra, else the cache
   cfft_internal::ScratchLease lease;
-  if (!scratch || scratch_bytes < bytes) {
+  if (!no_alloc && (!scratch || scratch_bytes < bytes)) {
     e = cfft_internal::scratch_acquire(bytes, s, &lease);
     if (e != hipSuccess) return hip_error(e, "external product scratch");
   }
@@ -190,7 +203,7 @@ int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int l
                           cfft_stream_t stream) {
   return cfft_internal::external_product_with_scratch(plan, key_fourier, k1, levels, base_log,
                                                       glwe, out, batch, (hipStream_t)stream,
-                                                      nullptr, 0);
+                                                      nullptr, 0, false);
 }
 
 }  // extern "C"

@@ -268,7 +268,7 @@ hipError_t cfft_server_s::launch(Batch &b) {
     const size_t io = round_up(b.in_bytes) + round_up(m * b.reqs[0].out_bytes);
     rc = cfft_internal::external_product_with_scratch(
         k.plan, k.d_fourier, k.k1, k.levels, k.base_log, reinterpret_cast<const uint64_t *>(d_in),
-        reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io);
+        reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io, false);
   } else {
     cfft_plan p = plans[b.log2N].load(std::memory_order_acquire);
     rc = cfft_negacyclic_polymul_u64(p, reinterpret_cast<const uint64_t *>(d_in),

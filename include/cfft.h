@@ -240,12 +240,26 @@ int cfft_ggsw_to_fourier(cfft_plan plan, const uint64_t *key, int k1, int levels
  * 2^(64 - levels*base_log), ties up; balanced digits in [-2^(B-1), 2^(B-1)),
  * level 0 most significant), forward transforms, Fourier multiply-accumulate
  * against the resident key (acc += key * digit, rows in order from acc = 0),
- * inverse, untwist, torus rounding.  1 <= base_log <= 32,
- * levels * base_log <= 64; the (1 + k1) Fourier buffers of one GLWE must fit
- * in one workgroup's LDS (k1 <= 3 at N = 4096).  Device buffers. */
+ * inverse, untwist, torus rounding.  1 <= k1 <= 64, 1 <= base_log <= 32,
+ * levels * base_log <= 64.  The digit spectra go through a device scratch
+ * taken from the library's stream-ordered cache (allocated on first use).
+ * Device buffers. */
 int cfft_external_product(cfft_plan plan, const void *key_fourier, int k1, int levels,
                           int base_log, const uint64_t *glwe, uint64_t *out, size_t batch,
                           cfft_stream_t stream);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation).  cfft_external_product with the same arguments and the
+ * same bits, allocating nothing: the spectra go through a fixed per-device
+ * scratch of cfft_noalloc_scratch_bytes() (32 MiB) held until the process
+ * ends, in passes of as many GLWEs as it holds (k1 * levels * N/2 * 16 bytes
+ * each); CFFT_ERROR_SIZE if one GLWE's spectra exceed it.  Every call on one
+ * device shares that device's scratch: calls that overlap on a device race
+ * (serialize them, e.g. on one stream). */
+int cfft_external_product_noalloc(cfft_plan plan, const void *key_fourier, int k1, int levels,
+                                  int base_log, const uint64_t *glwe, uint64_t *out,
+                                  size_t batch, cfft_stream_t stream);
+size_t cfft_noalloc_scratch_bytes(void);
 
 /* ---- uneven batch (no counterpart in the reference, which runs one plan
  *      per call): f64 negacyclic round trips of mixed N in one launch ---- */
