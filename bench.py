@@ -12,8 +12,9 @@ Printed JSON line: value = poly-mults/s of the whole job; roofline = the
 fused kernel's algorithmic HBM bytes (24 N bytes per product: A and B read,
 C written) per launch over its average launch time measured with HIP events
 on its stream; plain = the same products issued one call per polynomial;
-cpu_baseline = the oracle's restatement of the reference f64 path (one plan
-per thread over the host cores) on a bounded sample.
+cpu_baseline = the reference f64 path restated on the host (its AVX-512 and
+AVX2 SIMD forms and the scalar form, one pthread per host core, the fastest
+reported) on a bounded sample.
 
 Beside the headline, `uneven` reports the uneven batch: 20 000 f64
 negacyclic round-trip jobs (fold, twist, forward, inverse, untwist) with N
@@ -53,11 +54,26 @@ BATCH = 8192
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
-# HBM bytes per 8192-product launch of the persistent Dif16 kernel from
-# rocprofv3 --pmc: FETCH_SIZE x 2 (the counter reads half of a plain copy
-# kernel's known read size on this platform) + WRITE_SIZE (exact for a copy):
-# (131400 x 2 + 131078) KiB.
-TRAFFIC_BYTES = (131400 * 2 + 131078) * 1024
+# HBM bytes per 8192-product launch of the persistent Dif16 kernel, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over 4 launches: FETCH_SIZE x 2
+# (the counter reads half of a plain copy kernel's known read size on this
+# platform) + WRITE_SIZE (exact for a copy): (131390 x 2 + 131097) KiB.  The
+# figure holds for the kernel sources it was measured on (their SHA-256
+# below); when they change, `traffic` is reported as null until the counters
+# are run again.
+TRAFFIC_BYTES = (131390 * 2 + 131097) * 1024
+TRAFFIC_SOURCES = ("concrete-fft_amd/csrc/polymul_kernels.hip", "concrete-fft_amd/csrc/r16_exec.hpp",
+                   "concrete-fft_amd/csrc/device_math.hpp")
+TRAFFIC_SHA256 = "0fc3f872e6deac298c4adc9d05868b3a56c3e856b409896c58b165b606ed3cc9"
+
+
+def traffic_sources_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for f in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def host_cores():
@@ -74,26 +90,48 @@ def make_inputs(seed=2):
     return a, b
 
 
-def cpu_baseline(a, b, algo, base_n, target_s=10.0):
-    """Oracle restatement of the reference f64 path over all host cores: the
-    whole batch once (its result is compared with the GPU's), then repeated
-    passes until about target_s seconds of wall time."""
+CPU_SERIES = (("avx512", "or_simd_time_polymul_v4", 4), ("avx2", "or_simd_time_polymul_v3", 3),
+              ("scalar", "or_time_polymul", 0))
+
+
+def cpu_baseline(a, b, algo, base_n, target_s=4.0):
+    """The reference f64 path restated on the host, one plan shared by one
+    pthread per host core: the AVX-512 (pulp V4) and AVX2 (V3) series, both
+    bit-identical to the reference's golden vector, and the scalar series.
+    Each runs the whole batch once (its result is compared with the GPU's),
+    then repeated passes until about target_s seconds of wall time."""
     from oracle_lib import load_oracle
     O = load_oracle()
     threads = host_cores()
-    c = np.zeros((BATCH, N), dtype=np.uint64)
-    total_t = O.L.or_time_polymul(N, algo, base_n, a.ctypes.data, b.ctypes.data, c.ctypes.data,
-                                  BATCH, threads)
-    done = BATCH
-    scratch = np.zeros_like(c)
-    while total_t < target_s and done < 1024 * BATCH:
-        total_t += O.L.or_time_polymul(N, algo, base_n, a.ctypes.data, b.ctypes.data,
-                                       scratch.ctypes.data, BATCH, threads)
-        done += BATCH
-    return {"value": done / total_t, "unit": "poly-mults/s", "cores": threads, "kind": "port",
-            "sample": "%d products (%d passes over the N=2048 batch), %.1f s wall; oracle C "
-                      "restatement of unordered::Plan (scalar, bit-identical to the reference "
-                      "f64 path), one plan per thread" % (done, done // BATCH, total_t)}, c, BATCH
+    series, results = {}, {}
+    for name, fn, isa in CPU_SERIES:
+        if isa and not O.L.or_cpu_has_isa(isa):
+            series[name] = {"value": None, "note": "host CPU lacks this instruction set"}
+            continue
+        f = getattr(O.L, fn)
+        c = np.zeros((BATCH, N), dtype=np.uint64)
+        t = f(N, algo, base_n, a.ctypes.data, b.ctypes.data, c.ctypes.data, BATCH, threads)
+        if t < 0:
+            series[name] = {"value": None, "note": "plan not supported by this series"}
+            continue
+        done = BATCH
+        scratch = np.zeros_like(c)
+        while t < target_s and done < 1024 * BATCH:
+            t += f(N, algo, base_n, a.ctypes.data, b.ctypes.data, scratch.ctypes.data, BATCH,
+                   threads)
+            done += BATCH
+        series[name] = {"value": round(done / t, 1), "products": done, "seconds": round(t, 2)}
+        results[name] = c
+    best = max((k for k in results), key=lambda k: series[k]["value"])
+    same = all(np.array_equal(results[k], results[best]) for k in results)
+    return {"value": series[best]["value"], "unit": "poly-mults/s", "cores": threads,
+            "kind": "port", "isa": best,
+            "sample": "%d products (%d passes over the N=2048 batch), %.1f s wall; C "
+                      "restatement of unordered::Plan's %s path (bit-identical to the "
+                      "reference golden vector), one plan shared by %d pthreads"
+                      % (series[best]["products"], series[best]["products"] // BATCH,
+                         series[best]["seconds"], best, threads),
+            "series": series, "series_identical": same}, results[best], BATCH
 
 
 UNEVEN_JOBS = 20000
@@ -469,11 +507,15 @@ def main():
                  "identical_results": bool(np.array_equal(c_plain[:npl], c_gpu[:npl]))}
 
     achieved = BYTES_PER_PRODUCT * BATCH / (avg_ms * 1e-3) / 1e9
+    fresh = traffic_sources_sha() == TRAFFIC_SHA256
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-            "traffic": args.traffic_bytes,
+            "traffic": args.traffic_bytes if fresh else None,
             "kernel": "k_polymul (fused twist+2 fwd+pointwise+inv+torus)",
             "bytes_per_launch": BYTES_PER_PRODUCT * BATCH, "avg_launch_ms": round(avg_ms, 4)}
+    if not fresh:
+        roof["traffic_note"] = ("kernel sources changed since the HBM counter run: traffic not "
+                                "re-measured")
 
     out = {"metric": "negacyclic poly-mults/s at N=2048, batch 8192 (batched)",
            "value": round(value, 1), "unit": "poly-mults/s", "n_gpus": 1,
@@ -496,6 +538,7 @@ def main():
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
         out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
         out["gpu_equals_cpu_bitwise"] = bool(np.array_equal(c_cpu, c_gpu[:ns]))
         # torus error of both against the exact schoolbook on a few products
         from oracle_lib import load_oracle

@@ -132,6 +132,23 @@ int or_torus_to_fourier(size_t N, int base_algo, size_t base_n, const uint64_t *
 int or_external_product(size_t N, int k1, int levels, int base_log, const or_c64 *key_fourier,
                         int base_algo, size_t base_n, const uint64_t *glwe, uint64_t *out);
 
+/* ---- SIMD restatement of the reference's x86 path (fft64_simd.c): V3 =
+ *      AVX2 + FMA (c64x2), V4 = AVX-512F (c64x4); bit-identical to the scalar
+ *      functions above; DIF base algorithms, n >= 32 ---- */
+int or_cpu_has_isa(int isa); /* 3: AVX2 + FMA, 4: AVX-512F + AVX-512DQ */
+int or_simd_unordered_fwd_batch_v3(size_t n, int base_algo, size_t base_n, or_c64 *buf, size_t batch);
+int or_simd_unordered_inv_batch_v3(size_t n, int base_algo, size_t base_n, or_c64 *buf, size_t batch);
+int or_simd_polymul_f64_v3(size_t N, int base_algo, size_t base_n, const uint64_t *a,
+                           const int64_t *b, uint64_t *c, size_t batch, int threads);
+double or_simd_time_polymul_v3(size_t N, int base_algo, size_t base_n, const uint64_t *a,
+                               const int64_t *b, uint64_t *c, size_t batch, int threads);
+int or_simd_unordered_fwd_batch_v4(size_t n, int base_algo, size_t base_n, or_c64 *buf, size_t batch);
+int or_simd_unordered_inv_batch_v4(size_t n, int base_algo, size_t base_n, or_c64 *buf, size_t batch);
+int or_simd_polymul_f64_v4(size_t N, int base_algo, size_t base_n, const uint64_t *a,
+                           const int64_t *b, uint64_t *c, size_t batch, int threads);
+double or_simd_time_polymul_v4(size_t N, int base_algo, size_t base_n, const uint64_t *a,
+                               const int64_t *b, uint64_t *c, size_t batch, int threads);
+
 /* ---- CPU baseline timing helper: runs the f64 negacyclic product over
  *      `batch` polynomials with `threads` pthreads (one plan per thread)
  *      and returns seconds ---- */

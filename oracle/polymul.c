@@ -51,6 +51,14 @@ void or_twist_table(size_t N, c64 *twist) {
 /* from_torus: nearest multiple of 2^-64 of frac(x), as a u64.  fract in
  * [-1/2, 1/2]; the value 2^63 (fract == 1/2) wraps to the same torus element
  * 0x8000000000000000 as -2^63. */
+/* CPU features for the SIMD restatement (fft64_simd.c) */
+int or_cpu_has_isa(int isa) {
+  __builtin_cpu_init();
+  if (isa == 3) return __builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma");
+  if (isa == 4) return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512dq");
+  return 0;
+}
+
 uint64_t or_from_torus(double x) {
   double fract = x - round(x);
   double v = round(fract * 18446744073709551616.0);

@@ -47,6 +47,15 @@ class Oracle:
             "or_torus_to_fourier": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_size]),
             "or_external_product": (c_int, [c_size, c_int, c_int, c_int, c_vp, c_int, c_size, c_vp, c_vp]),
             "or_time_polymul": (ctypes.c_double, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
+            "or_cpu_has_isa": (c_int, [c_int]),
+            "or_simd_unordered_fwd_batch_v3": (c_int, [c_size, c_int, c_size, c_vp, c_size]),
+            "or_simd_unordered_inv_batch_v3": (c_int, [c_size, c_int, c_size, c_vp, c_size]),
+            "or_simd_polymul_f64_v3": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
+            "or_simd_time_polymul_v3": (ctypes.c_double, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
+            "or_simd_unordered_fwd_batch_v4": (c_int, [c_size, c_int, c_size, c_vp, c_size]),
+            "or_simd_unordered_inv_batch_v4": (c_int, [c_size, c_int, c_size, c_vp, c_size]),
+            "or_simd_polymul_f64_v4": (c_int, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
+            "or_simd_time_polymul_v4": (ctypes.c_double, [c_size, c_int, c_size, c_vp, c_vp, c_vp, c_size, c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -83,6 +92,26 @@ class Oracle:
         rc = self.L.or_unordered_inv_batch(n, int(base_algo), base_n, z.ctypes.data, z.size // n)
         assert rc == 0, rc
         return z
+
+    def simd_unordered(self, isa, z, base_algo, base_n, fwd=True):
+        """The V3 (isa 3) / V4 (isa 4) restatement of the reference's SIMD path."""
+        z = np.ascontiguousarray(z, dtype=np.complex128).copy()
+        n = z.shape[-1]
+        f = getattr(self.L, "or_simd_unordered_%s_batch_v%d" % ("fwd" if fwd else "inv", isa))
+        rc = f(n, int(base_algo), base_n, z.ctypes.data, z.size // n)
+        assert rc == 0, rc
+        return z
+
+    def simd_polymul_f64(self, isa, a, b, base_algo, base_n, threads=8):
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = np.ascontiguousarray(b, dtype=np.int64)
+        N = a.shape[-1]
+        c = np.zeros_like(a)
+        rc = getattr(self.L, "or_simd_polymul_f64_v%d" % isa)(
+            N, int(base_algo), base_n, a.ctypes.data, b.ctypes.data, c.ctypes.data, a.size // N,
+            threads)
+        assert rc == 0, rc
+        return c
 
     def ordered_fwd(self, z, algo):
         z = np.ascontiguousarray(z, dtype=np.complex128).copy()
