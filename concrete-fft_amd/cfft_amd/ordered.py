@@ -109,10 +109,10 @@ class _PlanBase:
 
 class Plan(_PlanBase):
     """ordered::Plan (src/ordered.rs:186-373): standard order in and out,
-    n <= 1024 (with large=True, n <= 8192)."""
+    n <= 1024 (with large=True, n <= 65536)."""
 
     def __init__(self, n, method, large=False):
-        """large=True admits 2^11 <= n <= 2^13 (cfft_ordered_plan_create_large):
+        """large=True admits 2^11 <= n <= 2^16 (cfft_ordered_plan_create_large):
         the reference's ordered recursion beyond its n < 2^11 assertion."""
         h = ctypes.c_void_p()
         create = lib.cfft_ordered_plan_create_large if large else lib.cfft_ordered_plan_create

@@ -17,6 +17,7 @@
 #include "host_tables.hpp"
 #include "launch.hpp"
 #include "plan.hpp"
+#include "scratch.hpp"
 
 using cfft_launch::dc64;
 using cfft_prog::Pass;
@@ -268,6 +269,37 @@ int build_f64_plan(cfft_plan_s *p) {
   return CFFT_SUCCESS;
 }
 
+// An ordered plan above one workgroup's LDS (2^13 < n <= 2^16): every pass
+// of the Stockham recursion over global memory.  Core passes move the data
+// between the caller's buffer and a scratch of batch x n points; the end
+// butterfly writes where it reads, so it runs in place, or from the scratch
+// back into the caller's buffer when the number of core passes is odd.
+int run_global_stockham(const cfft_plan_s *p, const std::vector<Pass> &passes, dc64 *d,
+                        size_t batch, size_t stride, hipStream_t s) {
+  const uint32_t log2n = ilog2(p->n);
+  size_t cores = 0;
+  for (const Pass &P : passes)
+    if (P.kind == cfft_prog::DIF_CORE || P.kind == cfft_prog::DIT_CORE) ++cores;
+  cfft_internal::ScratchLease lease;
+  CFFT_HIP(cfft_internal::scratch_acquire(batch * p->n * sizeof(dc64), s, &lease));
+  dc64 *cur = d, *oth = reinterpret_cast<dc64 *>(lease.ptr);
+  size_t cur_stride = stride, oth_stride = p->n;
+  hipError_t e = hipSuccess;
+  for (const Pass &P : passes) {
+    if (e != hipSuccess) break;
+    if (P.kind == cfft_prog::END && cores % 2 == 0) {
+      e = cfft_launch::pass_global(P, cur, cur_stride, cur, cur_stride, batch, log2n, p->d_tw, s);
+      continue;
+    }
+    e = cfft_launch::pass_global(P, cur, cur_stride, oth, oth_stride, batch, log2n, p->d_tw, s);
+    std::swap(cur, oth);
+    std::swap(cur_stride, oth_stride);
+  }
+  const hipError_t e2 = cfft_internal::scratch_release(lease, s);
+  if (e != hipSuccess) return hip_error(e, "global Stockham pass");
+  return hip_error(e2, "scratch release");
+}
+
 int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, bool fwd,
                   hipStream_t s) {
   if (!p) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
@@ -299,6 +331,10 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
     return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),
                      "w2048_transform");
+  const std::vector<Pass> &global = fwd ? p->fwd_global : p->inv_global;
+  if (!global.empty() && global.front().kind != cfft_prog::DEPTH_FWD &&
+      global.front().kind != cfft_prog::DEPTH_INV)
+    return run_global_stockham(p, global, d, batch, stride, s);
   if (fwd) {
     for (const Pass &P : p->fwd_global)
       CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
@@ -576,9 +612,8 @@ int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measur
   if (!out) return set_error(CFFT_ERROR_NULL_POINTER, "out is null");
   *out = nullptr;
   if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
-  // the whole transform runs in one workgroup's LDS chunk
-  if (ilog2(n) > cfft_launch::kLdsMaxLog2)
-    return set_error(CFFT_ERROR_SIZE, "ordered plans run up to n = 2^13 on the device");
+  // the range the reference documents for the ordered plan (src/ordered.rs:185)
+  if (ilog2(n) > 16) return set_error(CFFT_ERROR_SIZE, "ordered plans run up to n = 2^16");
   size_t base = n;
   if (method == CFFT_METHOD_MEASURE) {
     int rc = measure(cfft_plan_s::ORDERED, n, measure_seconds, &algo, &base);

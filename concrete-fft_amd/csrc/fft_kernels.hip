@@ -61,6 +61,32 @@ __global__ void __launch_bounds__(256) k_depth_global(c64 *__restrict__ data, ui
   for (int k = 0; k < R; ++k) gp[out[k]] = v[k];
 }
 
+// Any pass (Stockham core, end butterfly or depth pass) of a transform too
+// large for one workgroup's LDS, straight over global memory: one thread per
+// butterfly, reading src and writing dst (which may be src for the passes
+// whose outputs land where their inputs were: END and DEPTH_*).
+template <int R>
+__global__ void __launch_bounds__(256) k_pass_global(const c64 *src, uint64_t src_stride,
+                                                     c64 *dst, uint64_t dst_stride,
+                                                     uint64_t total, uint32_t log2n,
+                                                     const c64 *__restrict__ tw, Pass P) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  constexpr uint32_t lr = RadixLog<R>::value;
+  const uint64_t tr = gid >> (log2n - lr);
+  const uint32_t b = (uint32_t)(gid & ((1ull << (log2n - lr)) - 1));
+  const c64 *sp = src + tr * src_stride;
+  c64 *dp = dst + tr * dst_stride;
+  uint32_t in[R], out[R], twi[R];
+  pass_positions<R>(P, b, in, out, twi);
+  c64 v[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) v[k] = sp[in[k]];
+  apply_butterfly<R>(P, v, twi, tw);
+#pragma unroll
+  for (int k = 0; k < R; ++k) dp[out[k]] = v[k];
+}
+
 __global__ void k_monomial(c64 *__restrict__ buf, uint32_t log2n, uint64_t degree,
                            const uint32_t *__restrict__ freq, const c64 *__restrict__ tw) {
   const uint32_t n = 1u << log2n;
@@ -228,6 +254,38 @@ hipError_t depth_global(const Pass &P, dc64 *data, uint64_t batch, uint32_t log2
     default:
       hipLaunchKernelGGL(k_depth_global<8>, dim3((uint32_t)blocks), dim3(threads), 0, s, d, total,
                          log2n, stride, t, P);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t pass_global(const Pass &P, const dc64 *src, uint64_t src_stride, dc64 *dst,
+                       uint64_t dst_stride, uint64_t batch, uint32_t log2n, const dc64 *tw,
+                       hipStream_t s) {
+  const uint32_t lr = P.radix == 2 ? 1 : P.radix == 4 ? 2 : P.radix == 8 ? 3 : 4;
+  const uint64_t total = batch << (log2n - lr);
+  if (total == 0) return hipSuccess;
+  const uint32_t threads = 256;
+  const dim3 grid((uint32_t)((total + threads - 1) / threads));
+  const c64 *a = reinterpret_cast<const c64 *>(src);
+  c64 *d = reinterpret_cast<c64 *>(dst);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  switch (P.radix) {
+    case 2:
+      hipLaunchKernelGGL(k_pass_global<2>, grid, dim3(threads), 0, s, a, src_stride, d, dst_stride,
+                         total, log2n, t, P);
+      break;
+    case 4:
+      hipLaunchKernelGGL(k_pass_global<4>, grid, dim3(threads), 0, s, a, src_stride, d, dst_stride,
+                         total, log2n, t, P);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_pass_global<8>, grid, dim3(threads), 0, s, a, src_stride, d, dst_stride,
+                         total, log2n, t, P);
+      break;
+    default:
+      hipLaunchKernelGGL(k_pass_global<16>, grid, dim3(threads), 0, s, a, src_stride, d,
+                         dst_stride, total, log2n, t, P);
       break;
   }
   return hipGetLastError();

@@ -26,6 +26,12 @@ hipError_t fft_lds(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,
 hipError_t depth_global(const cfft_prog::Pass &P, dc64 *data, uint64_t batch, uint32_t log2n,
                         uint64_t stride, const dc64 *tw, hipStream_t s);
 
+// One pass of any kind over global memory, src -> dst (dst may equal src for
+// END and DEPTH_* passes, whose outputs land where their inputs were).
+hipError_t pass_global(const cfft_prog::Pass &P, const dc64 *src, uint64_t src_stride, dc64 *dst,
+                       uint64_t dst_stride, uint64_t batch, uint32_t log2n, const dc64 *tw,
+                       hipStream_t s);
+
 // buf[b*stride + i] = T[(freq[i] * degree) & (n-1)]
 hipError_t monomial(dc64 *buf, uint32_t log2n, uint64_t degree, const uint32_t *freq,
                     const dc64 *tw, hipStream_t s);

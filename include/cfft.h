@@ -105,14 +105,16 @@ int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms);
  * the device for about `measure_seconds` each (Method::Measure(Duration)). */
 int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seconds,
                              cfft_plan *out);
-/* The same ordered plan for 2^11 <= n <= 2^13 (and any smaller n).  The
+/* The same ordered plan for 2^11 <= n <= 2^16 (and any smaller n).  The
  * reference documents the ordered plan up to 2^16 (src/ordered.rs:185) but
  * Plan::new asserts n < 2^11 (src/ordered.rs:244), so no reference output
  * exists at these sizes: the plan runs the reference's ordered recursion
  * (Stockham cores of the chosen radix, same twiddle table) extended to n,
- * checked against the closed-form DFT.  n above 2^13 (one workgroup's LDS)
- * returns CFFT_ERROR_SIZE.  cfft_ordered_plan_create keeps the reference's
- * assertion. */
+ * checked against the closed-form DFT.  Up to 2^13 a transform runs in one
+ * workgroup's LDS; above, each pass runs over global memory through a device
+ * scratch of batch x n points from the library's stream-ordered cache.  n
+ * above 2^16 returns CFFT_ERROR_SIZE.  cfft_ordered_plan_create keeps the
+ * reference's assertion. */
 int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measure_seconds,
                                    cfft_plan *out);
 /* unordered::Plan::new(n, method), src/unordered.rs:659-747: n a power of

@@ -589,9 +589,9 @@ static int ordered_batch_max(size_t n, int algo, c64 *buf, size_t batch, int fwd
 /* Plan::new asserts n < 2^11 (src/ordered.rs:243-244) */
 int or_ordered_fwd_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 1, 1024); }
 int or_ordered_inv_batch(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 0, 1024); }
-/* The same recursion and twiddle tables without that assert, up to n = 2^13
- * (the plan's documented range reaches 2^16, src/ordered.rs:185).  No output
- * of the reference pins these sizes: they are checked against the closed-form
- * DFT and by round trips. */
-int or_ordered_fwd_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 1, 8192); }
-int or_ordered_inv_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 0, 8192); }
+/* The same recursion and twiddle tables without that assert, up to n = 2^16
+ * (the plan's documented range, src/ordered.rs:185).  No output of the
+ * reference pins these sizes: they are checked against the closed-form DFT
+ * and by round trips. */
+int or_ordered_fwd_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 1, 65536); }
+int or_ordered_inv_batch_large(size_t n, int algo, c64 *buf, size_t batch) { return ordered_batch_max(n, algo, buf, batch, 0, 65536); }

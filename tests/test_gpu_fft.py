@@ -51,9 +51,11 @@ def test_ordered_bit_exact(cfft, oracle, algo):
 
 @pytest.mark.parametrize("algo", ALGOS)
 def test_ordered_large_bit_exact(cfft, oracle, algo):
-    # cfft_ordered_plan_create_large: 2^11 <= n <= 2^13 against the oracle's
+    # cfft_ordered_plan_create_large: 2^11 <= n <= 2^16 against the oracle's
     # ordered recursion at the same sizes (itself pinned to the closed-form
     # DFT on the CPU)
+    from cfft_amd import CfftPanic, check, lib
+    from cfft_amd.device import DeviceBuffer
     from cfft_amd.ordered import FftAlgo, Method, Plan
     rng = np.random.default_rng(40 + algo)
     for n in (2048, 4096, 8192):
@@ -67,6 +69,28 @@ def test_ordered_large_bit_exact(cfft, oracle, algo):
         plan.inv(w)
         assert np.array_equal(w, oracle.ordered_inv(y, algo)), (algo, n)
         assert np.max(np.abs(w / n - x)) < 1e-14
+    # 2^14..2^16: every pass over global memory (ping-pong through a scratch;
+    # odd and even numbers of core passes), strided batches included
+    for n in (16384, 32768, 65536):
+        plan = Plan(n, Method.UserProvided(FftAlgo(algo)), large=True)
+        x = rand_c(rng, (2, n))
+        y = x.copy()
+        plan.fwd(y)
+        assert np.array_equal(y, oracle.ordered_fwd(x, algo)), (algo, n)
+        w = y.copy()
+        plan.inv(w)
+        assert np.array_equal(w, oracle.ordered_inv(y, algo)), (algo, n)
+        assert np.max(np.abs(w / n - x)) < 1e-13
+    n = 16384
+    plan = Plan(n, Method.UserProvided(FftAlgo(algo)), large=True)
+    x = rand_c(rng, (3, n + 64))
+    dx = DeviceBuffer.from_host(x)
+    check(lib.cfft_fwd(plan.handle, dx.dptr, 3, n + 64, None))
+    got = dx.download()
+    assert np.array_equal(got[:, :n], oracle.ordered_fwd(x[:, :n], algo))
+    assert np.array_equal(got[:, n:], x[:, n:])  # the gaps between transforms untouched
+    with pytest.raises(CfftPanic, match="2\\^16"):
+        Plan(1 << 17, Method.UserProvided(FftAlgo(algo)), large=True)
     # below 2^11 the large entry point builds the same plan as the capped one
     small = Plan(512, Method.UserProvided(FftAlgo(algo)), large=True)
     x = rand_c(rng, (2, 512))

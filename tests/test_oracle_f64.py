@@ -91,7 +91,7 @@ def test_ordered_large_vs_numpy(O, algo):
     # reference output exists, so the closed-form DFT (pocketfft) and the
     # round trip pin it; the capped entry keeps refusing these sizes
     rng = np.random.default_rng(200 + algo)
-    for n in (2048, 4096, 8192):
+    for n in (2048, 4096, 8192, 16384, 32768, 65536):
         x = rng.random((2, n)) + 1j * rng.random((2, n))
         y = O.ordered_fwd(x, algo)
         ref = np.fft.fft(x, axis=-1)
