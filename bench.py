@@ -426,6 +426,8 @@ def main():
                                        "transforms", "copy"], default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
+    ap.add_argument("--exact-steps", type=int, default=50)
+    ap.add_argument("--exact-warmup", type=int, default=10)
     ap.add_argument("--copy-mib", type=int, default=128)
     args = ap.parse_args()
 
@@ -453,7 +455,8 @@ def main():
         print(json.dumps({"mode": "serve", "serve": bench_serve(args.serve_seconds)}))
         return 0
     if args.mode == "exact":
-        print(json.dumps({"mode": "exact", "exact": bench_exact(50, 10)}))
+        print(json.dumps({"mode": "exact",
+                          "exact": bench_exact(args.exact_steps, args.exact_warmup)}))
         return 0
     if args.mode == "ordered":
         print(json.dumps({"mode": "ordered", "ordered": bench_ordered()}))
@@ -531,7 +534,7 @@ def main():
     if args.mode == "all":
         out["uneven"] = bench_uneven(args.uneven_steps, 2)
         out["serve"] = bench_serve(args.serve_seconds)
-        out["exact"] = bench_exact(50, 10)
+        out["exact"] = bench_exact(args.exact_steps, args.exact_warmup)
         out["ordered"] = bench_ordered()
         out["transforms"] = bench_transforms()
 

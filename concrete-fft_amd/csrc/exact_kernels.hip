@@ -38,6 +38,19 @@ __device__ __forceinline__ int64_t limb16(uint64_t v, uint32_t li) {
   return (int64_t)((e >> (16 * li)) & 0xffffu) - 32768;
 }
 
+// Nearest integer of x (|x| < 2^51) as a two's-complement u64, and the
+// distance |x - that integer|: x + 1.5 2^52 lies in [2^52, 2^53), where the
+// spacing is 1, so the addition rounds x to an integer (half to even) in the
+// low mantissa bits, and the two bit patterns differ by exactly that
+// integer.  (Differs from round-half-away only at exact half-integers, where
+// the distance is 1/2 and the product's margin reports it.)
+__device__ __forceinline__ uint64_t nearest_u64(double x, double &dist) {
+  constexpr double kMagic = 6755399441055744.0;  // 1.5 2^52
+  const double xm = x + kMagic;
+  dist = fabs(x - (xm - kMagic));
+  return __builtin_bit_cast(uint64_t, xm) - __builtin_bit_cast(uint64_t, kMagic);
+}
+
 template <int E>
 __global__ void __launch_bounds__(512) k_exact_fwd(const uint64_t *__restrict__ A,
                                                    const uint64_t *__restrict__ B,
@@ -98,10 +111,11 @@ __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
   };
   auto store = [&](uint32_t pos, c64 v) {
     const c64 r = cmul(untwist[pos], v);
-    const double x0 = round(r.re), x1 = round(r.im);
-    worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
-    part[pos] = (uint64_t)(int64_t)x0 << (16 * k);
-    part[n + pos] = (uint64_t)(int64_t)x1 << (16 * k);
+    double d0, d1;
+    const uint64_t x0 = nearest_u64(r.re, d0), x1 = nearest_u64(r.im, d1);
+    worst = fmax(worst, fmax(d0, d1));
+    part[pos] = x0 << (16 * k);
+    part[n + pos] = x1 << (16 * k);
   };
   run_program<E>(inv, lds, lt, T, tw, load, store);  // ends with a barrier
   const uint64_t *p0 = reinterpret_cast<const uint64_t *>(smem);
@@ -241,10 +255,11 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
     const uint32_t j = l + S::T * S::end_m(s);
     const c64 tz = tws[j];
     const c64 r = cmul(mk(tz.re * inv_n, (-tz.im) * inv_n), v[s]);
-    const double x0 = round(r.re), x1 = round(r.im);
-    worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
-    part[2 * s] = (uint64_t)(int64_t)x0 << (16 * k);
-    part[2 * s + 1] = (uint64_t)(int64_t)x1 << (16 * k);
+    double d0, d1;
+    const uint64_t x0 = nearest_u64(r.re, d0), x1 = nearest_u64(r.im, d1);
+    worst = fmax(worst, fmax(d0, d1));
+    part[2 * s] = x0 << (16 * k);
+    part[2 * s + 1] = x1 << (16 * k);
   }
   // the transform's last LDS reads are behind its final wave fence; reuse the
   // buffer for this wave's 32 words per lane
@@ -299,14 +314,17 @@ k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_blk[i];
   __syncthreads();
+  static_assert(kXW == 8, "one workgroup holds the 8 limb transforms of one product");
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint64_t u = (uint64_t)blockIdx.x * kXW + wave;  // 8 p + limb transform
-  if (u >= batch * 8) return;
+  // the grid is one workgroup per product, so every wave is active; the
+  // guard keeps every wave at the barriers below if that ever changes
+  const bool active = u < batch * 8;
   c64 *x = smem + kXTw + wave * S::lds_words;
   const uint32_t l = threadIdx.x & 63u;
-  const uint64_t p = u >> 3;
+  const uint64_t p = active ? u >> 3 : 0;
   const uint32_t li = (uint32_t)(u & 7), lb = li & 3;
-  if (li == 0 && l == 0 && margin) margin[p] = 0.0;
+  if (active && li == 0 && l == 0 && margin) margin[p] = 0.0;
   const uint64_t *src = (li < 4 ? A : B) + p * (2ull * n);
   c64 va[16], vb[16];
   // each half (points j and 1024 + j): its coefficient loads first, then
@@ -337,6 +355,7 @@ k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
   // b_{w-4} after), slot s of lane l at s * 64 + l; thread t forms the four
   // weights of buffer positions t and t + 512 (every wave of the workgroup
   // belongs to product p: the grid has one workgroup per product)
+  const bool store = (uint64_t)blockIdx.x < batch;
   c64 *W = F + p * 4ull * n;
   const c64 *xs = smem + kXTw;
   const uint32_t t = threadIdx.x;
@@ -362,7 +381,7 @@ k_exact_fwd_w2(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
         c64 d = cmul(fa[0], fb[k]);
 #pragma unroll
         for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(fa[i], fb[k - i]));
-        st_stream(d, &W[k * n + q]);
+        if (store) st_stream(d, &W[k * n + q]);
       }
     }
     __syncthreads();  // the buffers are rewritten for the second half
@@ -407,22 +426,29 @@ k_exact_inv_w2(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *marg
     for (uint32_t s = 0; s < 16; ++s) {
       const uint32_t q = h * 1024 + l + 64 * S::end_m(s);
       const c64 r = cmul(untwist[q], v[s]);
-      const double x0 = round(r.re), x1 = round(r.im);
-      worst = fmax(worst, fmax(fabs(r.re - x0), fabs(r.im - x1)));
-      xb[(2 * s) * 64 + l] = (uint64_t)(int64_t)x0 << (16 * k);
-      xb[(2 * s + 1) * 64 + l] = (uint64_t)(int64_t)x1 << (16 * k);
+      double d0, d1;
+      const uint64_t x0 = nearest_u64(r.re, d0), x1 = nearest_u64(r.im, d1);
+      worst = fmax(worst, fmax(d0, d1));
+      xb[(2 * s) * 64 + l] = x0 << (16 * k);
+      xb[(2 * s + 1) * 64 + l] = x1 << (16 * k);
       if ((s & 3) == 3) sched_fence();
     }
+    // the margin is reduced here, not sunk to its use at the end (which
+    // would keep the half's rounding residues live, and spilled)
+    asm volatile("" : "+v"(worst));
     __syncthreads();
     if (active) {
       // wave k adds and stores slots 4k .. 4k + 3 of the four weights' words
+      // (lane index recomputed: store addresses kept live across the first
+      // half would spill)
+      const uint32_t lh = launder_u32(threadIdx.x) & 63u;
       const uint64_t *b0 = reinterpret_cast<const uint64_t *>(x - k * S::lds_words);
       const uint32_t stride = S::lds_words * 2;  // one buffer, in words
 #pragma unroll
       for (uint32_t e = 0; e < 4; ++e) {
         const uint32_t s = 4 * k + e;
-        const uint32_t q = h * 1024 + l + 64 * S::end_m(s);
-        const uint32_t w0 = (2 * s) * 64 + l, w1 = (2 * s + 1) * 64 + l;
+        const uint32_t q = h * 1024 + lh + 64 * S::end_m(s);
+        const uint32_t w0 = (2 * s) * 64 + lh, w1 = (2 * s + 1) * 64 + lh;
         st_stream(b0[w0] + b0[stride + w0] + b0[2 * stride + w0] + b0[3 * stride + w0], &cp[q]);
         st_stream(b0[w1] + b0[stride + w1] + b0[2 * stride + w1] + b0[3 * stride + w1],
                   &cp[q + n]);

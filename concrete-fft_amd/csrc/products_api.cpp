@@ -163,15 +163,19 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
   if (plan->n < 2 || plan->n > 2048 || plan->fwd_full.npass == 0)
     return set_error(CFFT_ERROR_SIZE, "exact product supports 4 <= N <= 4096 (fft_size <= 2048)");
   hipStream_t s = (hipStream_t)stream;
-  const size_t bytes = batch * 8 * plan->n * sizeof(cfft_launch::dc64);
+  // the wave-local forms keep the 4 weight spectra of a product, the generic
+  // form its 8 limb spectra
+  const bool r16 = plan->n == 1024 && plan->d_tw_r16;           // Dif16, base_n == n
+  const bool w2 = !r16 && plan->n == 2048 && plan->d_tw_base1024;  // Dif16, base_n == 1024
+  const size_t bytes = batch * ((r16 || w2) ? 4 : 8) * plan->n * sizeof(cfft_launch::dc64);
   cfft_internal::ScratchLease lease;
   hipError_t e = cfft_internal::scratch_acquire(bytes, s, &lease);
   if (e != hipSuccess) return hip_error(e, "exact product scratch");
   void *F = lease.ptr;
-  if (plan->n == 1024 && plan->d_tw_r16)  // Dif16 plan, base_n == n
+  if (r16)
     e = cfft_launch::exact_polymul_r16(a, b, c, margin, batch, plan->d_tw_r16, plan->d_twist,
                                        reinterpret_cast<cfft_launch::dc64 *>(F), s);
-  else if (plan->n == 2048 && plan->d_tw_base1024)  // Dif16 plan, base_n == 1024
+  else if (w2)
     e = cfft_launch::exact_polymul_w2(a, b, c, margin, batch, plan->d_tw, plan->d_tw_base1024,
                                       plan->d_twist, plan->d_untwist,
                                       reinterpret_cast<cfft_launch::dc64 *>(F), s);

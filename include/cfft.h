@@ -217,8 +217,12 @@ int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t
 /* Exact C = A * B mod (X^N + 1, 2^64) for u64 A, B: balanced 16-bit limbs,
  * one f64 negacyclic transform per limb, per-weight Fourier products,
  * integer rounding and carries.  `margin` (optional, device, batch doubles)
- * receives max |x - round(x)| over the rounded products of each polynomial:
- * the result is exact while it stays < 0.5. */
+ * receives max |x - nearest integer| over the rounded products of each
+ * polynomial: the result is exact while it stays < 0.5.  The call does not
+ * inspect the margins (that would synchronise the stream): a caller that
+ * needs the guarantee passes `margin` and checks it; for 4 <= N <= 4096 it
+ * stays far below 0.5 (measured 0.0098 at N = 4096 with every limb at
+ * -2^15, the largest |D_3|). */
 int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b, uint64_t *c,
                            double *margin, size_t batch, cfft_stream_t stream);
 /* The same exact product over the double-double transform of an fft128 plan

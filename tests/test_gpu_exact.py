@@ -136,3 +136,20 @@ def test_exact_calls_on_two_streams_share_scratch(cfft, oracle):
         s.synchronize()
     for a, b, _, _, dc in calls:
         assert np.array_equal(dc.download(), oracle.schoolbook_u64(a, b))
+
+
+@pytest.mark.parametrize("N,base_n", [(4096, 1024), (2048, 1024), (1024, 512)])
+def test_exact_worst_case_limbs(cfft, oracle, N, base_n):
+    # every coefficient 0x8000800080008000: all four balanced limbs are
+    # -2^15, so |D_3| reaches 4 N 2^30 and the rounding margin is tightest
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    v = 0x8000800080008000
+    a = np.full((2, N), v, np.uint64)
+    b = np.full((2, N), v, np.uint64)
+    b[1, ::3] = M - v  # limbs of opposite sign on every third coefficient
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, base_n))
+    c, margin = products.exact_polymul_u64_host(plan, a, b, with_margin=True)
+    assert np.array_equal(c, oracle.schoolbook_u64(a, b))
+    # exact while < 1/2; measured 0.0098 at N = 4096, so wide headroom remains
+    assert 0.0 <= margin.max() < 2.0 ** -5, margin
