@@ -397,6 +397,127 @@ def bench_ordered(steps=200):
                         "buffer, HIP events)", "plans": out}
 
 
+def bench_host_path(calls=2000):
+    """The drop-in host path: unordered::Plan::fwd / inv on a host buffer of
+    one n = 1024 polynomial per call (src/unordered.rs:826-839, the call a
+    TFHE-rs caller makes per polynomial), wall-clock latency per call (p50,
+    p99) through cfft_fwd_host / cfft_inv_host, checked bit-equal to the
+    device-buffer path; beside it the oracle's SIMD restatement of the same
+    plan on one host core (per transform, from a batch of 4096)."""
+    from oracle_lib import load_oracle
+    from cfft_amd import check, lib
+    from cfft_amd.device import DeviceBuffer
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    n, algo, base = 1024, FftAlgo.Dif16, 1024
+    plan = Plan(n, Method.UserProvided(algo, base))
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    out = {}
+    for name, f in (("fwd", lib.cfft_fwd_host), ("inv", lib.cfft_inv_host)):
+        buf = x.copy()
+        ptr_ = buf.ctypes.data
+        for _ in range(100):
+            buf[:] = x
+            check(f(plan.handle, ptr_, 1))
+        lat = []
+        for _ in range(calls):
+            t0 = time.perf_counter()
+            check(f(plan.handle, ptr_, 1))
+            lat.append(time.perf_counter() - t0)
+        buf[:] = x
+        check(f(plan.handle, ptr_, 1))
+        d = DeviceBuffer.from_host(x)
+        (plan.fwd if name == "fwd" else plan.inv)(d)
+        lat = np.array(lat) * 1e6
+        out[name] = {"p50_us": round(float(np.percentile(lat, 50)), 2),
+                     "p99_us": round(float(np.percentile(lat, 99)), 2),
+                     "calls": calls,
+                     "equals_device_path": bool(np.array_equal(buf.view(np.uint64),
+                                                               d.download().view(np.uint64)))}
+    o = load_oracle()
+    isa = 4 if o.L.or_cpu_has_isa(4) else 3
+    cpu = {}
+    for name, fwd in (("fwd", True), ("inv", False)):
+        z = np.tile(x, (4096, 1))
+        t0 = time.perf_counter()
+        o.simd_unordered(isa, z, int(algo), base, fwd=fwd)
+        cpu[name + "_us"] = round((time.perf_counter() - t0) / 4096 * 1e6, 3)
+    return {"workload": "unordered n=1024 Dif16/1024, host buffer, batch 1 per call",
+            "gpu": out, "cpu_one_core": dict(cpu, isa="avx512" if isa == 4 else "avx2",
+                                             kind="port")}
+
+
+def bench_matrix(steps=100):
+    """The reference's bench matrix (benches/fft.rs:116-207, :211-338) on
+    device buffers: for n = 2^8 .. 2^16, the unordered plan chosen by
+    Method::Measure(10 ms) -- forward, inverse and fwd_monomial at batch 1
+    (device time per call, median of `steps` HIP event pairs) and forward /
+    inverse batched over 8 Mi points per call (transforms/s, fraction of
+    8 TB/s at 32 n bytes each) -- and the ordered plan (n <= 1024, Measure,
+    batch-1 forward); for fft128 n = 64 .. 16384, the negacyclic forward and
+    inverse (batch 1 and batched over 4 Mi points)."""
+    import cfft_amd
+    from cfft_amd import check, fft128, lib, ordered
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import Method, Plan
+    s = Stream()
+    rng = np.random.default_rng(9)
+    big = DeviceBuffer.from_host(rng.uniform(-1, 1, 1 << 24).view(np.complex128), s)
+    rows = {}
+    for lg in range(8, 17):
+        n = 1 << lg
+        plan = Plan(n, Method.Measure(0.01))
+        algo, base = plan.algo()
+        one = DeviceBuffer.from_host(rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n), s)
+        row = {"plan": "%s/%d" % (algo.name, base)}
+        for name, f in (("fwd", lib.cfft_fwd), ("inv", lib.cfft_inv)):
+            ms = time_runs(lambda: check(f(plan.handle, one.dptr, 1, n, s.handle)), s, steps, 10)
+            row[name + "_us"] = round(float(np.median(ms)) * 1e3, 2)
+            batch = (1 << 23) // n
+            msb = time_region(lambda: check(f(plan.handle, big.dptr, batch, n, s.handle)), s,
+                              20, 3)
+            row[name + "_batched_per_s"] = round(batch / (msb * 1e-3), 1)
+            row[name + "_batched_hbm_frac"] = round(32 * n * batch / (msb * 1e-3) / 1e9
+                                                    / PEAK_HBM_GBS, 4)
+        deg = [0]
+
+        def mono():
+            deg[0] = (deg[0] + 1) % n
+            check(lib.cfft_fwd_monomial(plan.handle, deg[0], one.dptr, s.handle))
+        ms = time_runs(mono, s, steps, 10)
+        row["fwd_monomial_us"] = round(float(np.median(ms)) * 1e3, 2)
+        if n <= 1024:
+            op = ordered.Plan(n, ordered.Method.Measure(0.01))
+            ms = time_runs(lambda: op.fwd(one, s), s, steps, 10)
+            row["ordered_fwd_us"] = round(float(np.median(ms)) * 1e3, 2)
+            row["ordered_plan"] = op.algo().name
+        rows[str(n)] = row
+    r128 = {}
+    for lg in range(6, 15):
+        n = 1 << lg
+        row = {}
+        tw = [DeviceBuffer.from_host(t, s) for t in fft128.init_negacyclic_twiddles(n)]
+        d1 = [DeviceBuffer.from_host(rng.standard_normal(n), s) for _ in range(4)]
+        try:
+            fft128.negacyclic_fwd_fft(*d1, tw, 1, 0, s)
+        except cfft_amd.CfftPanic as e:  # sizes the device transforms do not take
+            r128[str(n)] = {"unsupported": str(e).splitlines()[0]}
+            continue
+        batch = (1 << 22) // n
+        db = [DeviceBuffer.from_host(rng.standard_normal(batch * n), s) for _ in range(4)]
+        for name, f in (("fwd", fft128.negacyclic_fwd_fft), ("inv", fft128.negacyclic_inv_fft)):
+            ms = time_runs(lambda: f(*d1, tw, 1, 0, s), s, steps, 10)
+            row[name + "_us"] = round(float(np.median(ms)) * 1e3, 2)
+            msb = time_region(lambda: f(*db, tw, batch, 0, s), s, 20, 3)
+            row[name + "_batched_per_s"] = round(batch / (msb * 1e-3), 1)
+            row[name + "_batched_hbm_frac"] = round(64 * n * batch / (msb * 1e-3) / 1e9
+                                                    / PEAK_HBM_GBS, 4)
+        r128[str(n)] = row
+    return {"workload": "reference bench matrix on device buffers (batch-1 device time per "
+                        "call; batched: 8 Mi c64 / 4 Mi f128 complex points per call)",
+            "unordered": rows, "fft128": r128}
+
+
 def copy_probe(mib):
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -423,7 +544,7 @@ def main():
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
     ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered",
-                                       "transforms", "copy"], default="all")
+                                       "transforms", "matrix", "host", "copy"], default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--exact-steps", type=int, default=50)
@@ -463,6 +584,12 @@ def main():
         return 0
     if args.mode == "transforms":
         print(json.dumps({"mode": "transforms", "transforms": bench_transforms()}))
+        return 0
+    if args.mode == "matrix":
+        print(json.dumps({"mode": "matrix", "matrix": bench_matrix()}))
+        return 0
+    if args.mode == "host":
+        print(json.dumps({"mode": "host", "host_path": bench_host_path()}))
         return 0
     a, b = make_inputs()
     plan = Plan(N // 2, Method.UserProvided(FftAlgo(args.algo), args.base_n))
@@ -537,6 +664,8 @@ def main():
         out["exact"] = bench_exact(args.exact_steps, args.exact_warmup)
         out["ordered"] = bench_ordered()
         out["transforms"] = bench_transforms()
+        out["matrix"] = bench_matrix()
+        out["host_path"] = bench_host_path()
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)

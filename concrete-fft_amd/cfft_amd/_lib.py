@@ -128,6 +128,14 @@ _SIGS = [
     ("cfft_server_release", c_int, [c_vp, c_u64]),
     ("cfft_server_replay", c_int, [c_vp, ctypes.POINTER(Request), c_size, c_vp, c_vp]),
     ("cfft_server_stats", c_int, [c_vp, ctypes.POINTER(ServerStatistics)]),
+    ("cfft_tune_cache_path", c_int, [ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
+    ("cfft_tune_device_key", c_int, [ctypes.c_char_p, c_size, ctypes.POINTER(c_size)]),
+    ("cfft_tune_cache_import", c_int, [ctypes.c_char_p, ctypes.POINTER(c_size)]),
+    ("cfft_tune_cache_export", c_int, [ctypes.c_char_p]),
+    ("cfft_tune_cache_clear", c_int, []),
+    ("cfft_tune_cache_size", c_int, [ctypes.POINTER(c_size)]),
+    ("cfft_tune_cache_lookup", c_int, [ctypes.c_char_p, c_int, c_size, ctypes.POINTER(c_int),
+                                       ctypes.POINTER(c_int), ctypes.POINTER(c_size)]),
     ("cfft_uneven_info", c_int, [c_vp, ctypes.POINTER(c_size), ctypes.POINTER(c_size),
                                  ctypes.POINTER(c_size), ctypes.POINTER(ctypes.c_uint32)]),
 ]

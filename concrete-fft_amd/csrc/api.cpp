@@ -17,7 +17,9 @@
 #include "host_tables.hpp"
 #include "launch.hpp"
 #include "plan.hpp"
+#include "host_stage.hpp"
 #include "scratch.hpp"
+#include "tune_cache.hpp"
 
 using cfft_launch::dc64;
 using cfft_prog::Pass;
@@ -63,6 +65,9 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_untwist) (void)hipFree(d_untwist);
   if (d_tw128) (void)hipFree(d_tw128);
 }
+
+int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
+                          size_t batch, hipStream_t s);
 
 namespace {
 
@@ -351,56 +356,80 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   return CFFT_SUCCESS;
 }
 
+// Plan::fwd / Plan::inv on a host buffer (src/unordered.rs:826-839,
+// src/ordered.rs:342-372): staged through a pinned slot (host_stage.hpp)
 int run_transform_host(const cfft_plan_s *p, double *buf, size_t batch, bool fwd) {
   if (!p) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
   if (batch == 0) return CFFT_SUCCESS;
   if (!buf) return set_error(CFFT_ERROR_NULL_POINTER, "buffer is null");
-  const size_t bytes = p->n * batch * sizeof(dc64);
-  void *d = nullptr;
-  CFFT_HIP(hipMalloc(&d, bytes));
-  int rc = hip_error(hipMemcpy(d, buf, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-  if (rc == CFFT_SUCCESS) rc = run_transform(p, d, batch, p->n, fwd, nullptr);
-  if (rc == CFFT_SUCCESS) rc = hip_error(hipMemcpy(buf, d, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
-  (void)hipFree(d);
-  return rc;
+  const size_t n = p->n, per = n * sizeof(dc64);
+  return cfft_internal::stage_host(
+      batch, per,
+      [&](void *pin, size_t b0, size_t nb) { std::memcpy(pin, buf + 2 * n * b0, nb * per); },
+      [&](void *d, size_t nb, hipStream_t s) { return run_transform(p, d, nb, n, fwd, s); },
+      [&](const void *pin, size_t b0, size_t nb) { std::memcpy(buf + 2 * n * b0, pin, nb * per); });
 }
 
 // ---- Method::Measure: time candidate plans on the device ----
-double time_plan(cfft_plan_s *p, double seconds) {
-  const size_t n = p->n;
-  const size_t target_bytes = (size_t)32 << 20;
-  size_t batch = std::max<size_t>(1, target_bytes / (n * sizeof(dc64)));
-  void *d = nullptr;
-  if (hipMalloc(&d, batch * n * sizeof(dc64)) != hipSuccess) return 1e30;
-  (void)hipMemset(d, 0, batch * n * sizeof(dc64));
-  hipEvent_t a, b;
-  (void)hipEventCreate(&a);
-  (void)hipEventCreate(&b);
-  // warm up
-  run_transform(p, d, batch, n, true, nullptr);
-  (void)hipDeviceSynchronize();
-  double best = 1e30;
-  auto t0 = std::chrono::steady_clock::now();
-  int reps = 0;
-  do {
-    (void)hipEventRecord(a, nullptr);
-    run_transform(p, d, batch, n, true, nullptr);
-    (void)hipEventRecord(b, nullptr);
-    (void)hipEventSynchronize(b);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, a, b);
-    best = std::min(best, (double)ms / (double)batch);
-    ++reps;
-  } while (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() < seconds &&
-           reps < 1000);
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  (void)hipFree(d);
-  return best;
+// Timing buffers (zero-filled): the transforms run on a batch of
+// kTuneTransformBytes, the fused product on a batch whose inputs and output
+// fill kTuneProductBytes.
+constexpr size_t kTuneTransformBytes = (size_t)32 << 20;
+constexpr size_t kTuneProductBytes = (size_t)96 << 20;
+
+struct TuneRig {
+  hipStream_t s = nullptr;  // private, non-blocking: the caller's streams run on
+  hipEvent_t a = nullptr, b = nullptr;
+  void *buf = nullptr;
+  size_t bytes = 0;
+};
+
+bool tune_product(const cfft_plan_s *p) {
+  return p->kind == cfft_plan_s::UNORDERED && p->n >= 2 && p->fwd_full.npass != 0;
 }
 
-std::mutex g_tune_mu;
-std::map<std::tuple<int, int, size_t>, std::pair<int, size_t>> g_tune_cache;
+// Times one candidate on each shape a plan of its kind serves: the forward
+// and the inverse transform and, for unordered plans the fused negacyclic
+// product runs on (n <= 8192), that product.  Rounds of one launch per shape
+// repeat until `seconds` have passed (at least 2 rounds, at most 1000); the
+// score is the sum over the shapes of the best time per polynomial.
+int time_plan(const cfft_plan_s *p, double seconds, const TuneRig &r, double *score) {
+  const size_t n = p->n;
+  const size_t tbatch = std::max<size_t>(1, kTuneTransformBytes / (n * sizeof(dc64)));
+  const bool prod = tune_product(p) && r.bytes >= kTuneProductBytes;
+  const size_t pbatch = prod ? std::max<size_t>(1, kTuneProductBytes / (6 * n * sizeof(uint64_t))) : 0;
+  const uint64_t *pa = static_cast<const uint64_t *>(r.buf);
+  const int64_t *pb = reinterpret_cast<const int64_t *>(pa + 2 * n * pbatch);
+  uint64_t *pc = const_cast<uint64_t *>(pa) + 4 * n * pbatch;
+  auto launch = [&](int shape) -> int {
+    if (shape < 2) return run_transform(p, r.buf, tbatch, n, shape == 0, r.s);
+    return cfft_polymul_dispatch(p, pa, pb, pc, pbatch, r.s);
+  };
+  const int shapes = prod ? 3 : 2;
+  for (int k = 0; k < shapes; ++k) {  // warm-up
+    int rc = launch(k);
+    if (rc) return rc;
+  }
+  CFFT_HIP(hipStreamSynchronize(r.s));
+  double best[3] = {1e30, 1e30, 1e30};
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int round = 0; round < 1000; ++round) {
+    for (int k = 0; k < shapes; ++k) {
+      CFFT_HIP(hipEventRecord(r.a, r.s));
+      int rc = launch(k);
+      if (rc) return rc;
+      CFFT_HIP(hipEventRecord(r.b, r.s));
+      CFFT_HIP(hipEventSynchronize(r.b));
+      float ms = 0;
+      CFFT_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      best[k] = std::min(best[k], (double)ms / (double)(k < 2 ? tbatch : pbatch));
+    }
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (round >= 1 && el >= seconds) break;
+  }
+  *score = best[0] + best[1] + (prod ? best[2] : 0.0);
+  return CFFT_SUCCESS;
+}
 
 int create_f64(cfft_plan_s::Kind kind, size_t n, int algo, size_t base_n, cfft_plan *out) {
   auto *p = new (std::nothrow) cfft_plan_s();
@@ -419,17 +448,51 @@ int create_f64(cfft_plan_s::Kind kind, size_t n, int algo, size_t base_n, cfft_p
   return CFFT_SUCCESS;
 }
 
+// device key of a tuning-cache entry: model, ISA, compute units, HIP
+// runtime and ABI version
+int device_key(int dev, std::string *out) {
+  static std::mutex mu;
+  static std::map<int, std::string> keys;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = keys.find(dev);
+  if (it == keys.end()) {
+    hipDeviceProp_t prop;
+    CFFT_HIP(hipGetDeviceProperties(&prop, dev));
+    int rt = 0;
+    CFFT_HIP(hipRuntimeGetVersion(&rt));
+    char buf[640];
+    snprintf(buf, sizeof buf, "%s|%s|cu%d|hip%d|abi%d", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount, rt, CFFT_ABI_VERSION);
+    std::string k(buf);
+    for (char &c : k)
+      if (c == '\t' || c == '\n' || c == '\r') c = ' ';
+    it = keys.emplace(dev, k).first;
+  }
+  *out = it->second;
+  return CFFT_SUCCESS;
+}
+
+// Method::Measure (src/unordered.rs:553-640, src/ordered.rs:99-180): the
+// fastest of the candidate (algo, base_n) pairs, each timed for about
+// `seconds` on the shapes the plan serves (time_plan).  Bases: n for ordered
+// plans and n <= 256, else 512 and 1024 (those not above n), with all 8
+// algorithms each.  The choice is kept in the tuning cache (tune_cache.hpp),
+// so a later Measure of the same kind and size on the same device and stack,
+// in this process or another, reuses it without timing.  Every HIP failure
+// is returned.
 int measure(cfft_plan_s::Kind kind, size_t n, double seconds, int *algo, size_t *base_n) {
   int dev = 0;
-  (void)hipGetDevice(&dev);
-  {
-    std::lock_guard<std::mutex> g(g_tune_mu);
-    auto it = g_tune_cache.find({dev, (int)kind, n});
-    if (it != g_tune_cache.end()) {
-      *algo = it->second.first;
-      *base_n = it->second.second;
-      return CFFT_SUCCESS;
-    }
+  CFFT_HIP(hipGetDevice(&dev));
+  std::string key;
+  int rc = device_key(dev, &key);
+  if (rc) return rc;
+  const int tk = kind == cfft_plan_s::ORDERED ? cfft_internal::TUNE_ORDERED
+                                              : cfft_internal::TUNE_UNORDERED;
+  cfft_internal::TuneChoice hit;
+  if (cfft_internal::tune_lookup(key, tk, n, &hit)) {
+    *algo = hit.algo;
+    *base_n = hit.base_n;
+    return CFFT_SUCCESS;
   }
   std::vector<size_t> bases;
   if (kind == cfft_plan_s::ORDERED || n <= 256) {
@@ -439,26 +502,49 @@ int measure(cfft_plan_s::Kind kind, size_t n, double seconds, int *algo, size_t 
       if (b <= n) bases.push_back(b);
   }
   const double per = std::max(1e-3, seconds);
+  TuneRig r;
+  const bool prod = kind == cfft_plan_s::UNORDERED && n >= 2 && n <= 8192;
+  r.bytes = prod ? kTuneProductBytes : std::max(kTuneTransformBytes, n * sizeof(dc64));
+  hipError_t e = hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&r.a);
+  if (e == hipSuccess) e = hipEventCreate(&r.b);
+  if (e == hipSuccess) e = hipMallocAsync(&r.buf, r.bytes, r.s);
+  if (e == hipSuccess) e = hipMemsetAsync(r.buf, 0, r.bytes, r.s);
+  rc = hip_error(e, "autotuner set-up");
   double best = 1e30;
   int best_algo = CFFT_DIF4;
   size_t best_base = bases.front();
   for (size_t b : bases) {
-    for (int a = 0; a < 8; ++a) {
+    for (int a = 0; a < 8 && rc == CFFT_SUCCESS; ++a) {
       cfft_plan p0 = nullptr;
-      if (create_f64(kind, n, a, b, &p0) != CFFT_SUCCESS) continue;
-      const double t = time_plan(p0, per);
+      rc = create_f64(kind, n, a, b, &p0);
+      if (rc) break;
+      double t = 0;
+      rc = time_plan(p0, per, r, &t);
       delete p0;
-      if (t < best) {
+      if (rc == CFFT_SUCCESS && t < best) {
         best = t;
         best_algo = a;
         best_base = b;
       }
     }
   }
+  // tear-down on the private stream only; a failure here is reported when
+  // nothing failed before it
+  e = hipSuccess;
+  if (r.buf) e = hipFreeAsync(r.buf, r.s);
+  if (r.s) {
+    const hipError_t e2 = hipStreamSynchronize(r.s);
+    if (e == hipSuccess) e = e2;
+  }
+  if (r.a) (void)hipEventDestroy(r.a);
+  if (r.b) (void)hipEventDestroy(r.b);
+  if (r.s) (void)hipStreamDestroy(r.s);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_error(e, "autotuner tear-down");
   *algo = best_algo;
   *base_n = best_base;
-  std::lock_guard<std::mutex> g(g_tune_mu);
-  g_tune_cache[{dev, (int)kind, n}] = {best_algo, best_base};
+  cfft_internal::tune_store(key, tk, n, cfft_internal::TuneChoice{best_algo, best_base});
   return CFFT_SUCCESS;
 }
 
@@ -480,9 +566,6 @@ void put_varint(std::vector<uint8_t> &o, uint64_t v) {
 }
 
 }  // namespace
-
-int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
-                          size_t batch, hipStream_t s);
 
 extern "C" {
 
@@ -587,6 +670,20 @@ int cfft_event_synchronize(cfft_event_t ev) {
 int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms) {
   if (!ms) return set_error(CFFT_ERROR_NULL_POINTER, "ms is null");
   return hip_error(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+// ---- tuning cache: the device key of the current device ----
+int cfft_tune_device_key(char *buf, size_t cap, size_t *len) {
+  int dev = 0;
+  CFFT_HIP(hipGetDevice(&dev));
+  std::string k;
+  int rc = device_key(dev, &k);
+  if (rc) return rc;
+  if (len) *len = k.size();
+  if (!buf) return cap ? set_error(CFFT_ERROR_NULL_POINTER, "buf is null") : CFFT_SUCCESS;
+  if (cap <= k.size()) return set_error(CFFT_ERROR_SIZE, "buffer too small for the key");
+  std::memcpy(buf, k.c_str(), k.size() + 1);
+  return CFFT_SUCCESS;
 }
 
 // ---- plans ----
@@ -702,12 +799,11 @@ int cfft_fwd_monomial(cfft_plan plan, size_t degree, void *buf, cfft_stream_t st
 }
 int cfft_fwd_monomial_host(cfft_plan plan, size_t degree, double *buf) {
   if (!plan || !buf) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
-  void *d = nullptr;
-  CFFT_HIP(hipMalloc(&d, plan->n * sizeof(dc64)));
-  int rc = cfft_fwd_monomial(plan, degree, d, nullptr);
-  if (rc == CFFT_SUCCESS) rc = hip_error(hipMemcpy(buf, d, plan->n * sizeof(dc64), hipMemcpyDeviceToHost), "hipMemcpy");
-  (void)hipFree(d);
-  return rc;
+  const size_t per = plan->n * sizeof(dc64);
+  return cfft_internal::stage_host(
+      1, per, [](void *, size_t, size_t) {},  // the kernel writes every point
+      [&](void *d, size_t, hipStream_t s) { return cfft_fwd_monomial(plan, degree, d, s); },
+      [&](const void *pin, size_t, size_t) { std::memcpy(buf, pin, per); });
 }
 
 // ---- serde ----

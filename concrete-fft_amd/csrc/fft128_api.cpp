@@ -2,10 +2,12 @@
 // src/fft128/mod.rs:1830-1961, and its #[doc(hidden)] free functions).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <new>
 #include <vector>
 
 #include "cfft.h"
+#include "host_stage.hpp"
 #include "host_tables.hpp"
 #include "plan.hpp"
 #include "scratch.hpp"
@@ -39,7 +41,7 @@ int run128(size_t n, bool fwd, int mulv, double *re0, double *re1, double *im0, 
     return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
   if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
   const uint32_t L = ilog2(n);
-  if (L < 3 || L > 12) return set_error(CFFT_ERROR_SIZE, "fft128 device transforms support 8 <= n <= 4096");
+  if (L < 3 || L > 20) return set_error(CFFT_ERROR_SIZE, "fft128 device transforms support 8 <= n <= 2^20");
   return hip_error(cfft_launch::fft128_run(fwd, mulv, L, re0, re1, im0, im1, t0, t1, t2, t3,
                                            batch, s),
                    "fft128");
@@ -55,6 +57,9 @@ int run128_plan(cfft_plan p, bool fwd, double *re0, double *re1, double *im0, do
   return run128(n, fwd, 0, re0, re1, im0, im1, t, t + n, t + 2 * n, t + 3 * n, batch, s);
 }
 
+// fft128 Plan::fwd / Plan::inv on host arrays (src/fft128/mod.rs:1905-1960),
+// staged through a pinned slot (host_stage.hpp): a chunk of nb transforms
+// sits in the slot as [re0 | re1 | im0 | im1], nb * n doubles each
 int run128_host(cfft_plan p, bool fwd, double *re0, double *re1, double *im0, double *im1,
                 size_t batch) {
   if (!p) return set_error(CFFT_ERROR_NULL_POINTER, "plan is null");
@@ -62,21 +67,24 @@ int run128_host(cfft_plan p, bool fwd, double *re0, double *re1, double *im0, do
     return set_error(CFFT_ERROR_WRONG_PLAN_KIND, "not an fft128 plan");
   if (batch == 0) return CFFT_SUCCESS;
   if (!re0 || !re1 || !im0 || !im1) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
-  const size_t bytes = p->n * batch * sizeof(double);
-  double *d = nullptr;
-  hipError_t e = hipMalloc(&d, 4 * bytes);
-  if (e != hipSuccess) return hip_error(e, "hipMalloc");
+  const size_t n = p->n;
   double *h[4] = {re0, re1, im0, im1};
-  int rc = CFFT_SUCCESS;
-  for (int i = 0; i < 4 && rc == CFFT_SUCCESS; ++i)
-    rc = hip_error(hipMemcpy(d + i * p->n * batch, h[i], bytes, hipMemcpyHostToDevice), "hipMemcpy");
-  if (rc == CFFT_SUCCESS)
-    rc = run128_plan(p, fwd, d, d + p->n * batch, d + 2 * p->n * batch, d + 3 * p->n * batch,
-                     batch, nullptr);
-  for (int i = 0; i < 4 && rc == CFFT_SUCCESS; ++i)
-    rc = hip_error(hipMemcpy(h[i], d + i * p->n * batch, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
-  (void)hipFree(d);
-  return rc;
+  return cfft_internal::stage_host(
+      batch, 4 * n * sizeof(double),
+      [&](void *pin, size_t b0, size_t nb) {
+        double *d = static_cast<double *>(pin);
+        for (int i = 0; i < 4; ++i)
+          std::memcpy(d + i * n * nb, h[i] + n * b0, n * nb * sizeof(double));
+      },
+      [&](void *dv, size_t nb, hipStream_t s) {
+        double *d = static_cast<double *>(dv);
+        return run128_plan(p, fwd, d, d + n * nb, d + 2 * n * nb, d + 3 * n * nb, nb, s);
+      },
+      [&](const void *pin, size_t b0, size_t nb) {
+        const double *d = static_cast<const double *>(pin);
+        for (int i = 0; i < 4; ++i)
+          std::memcpy(h[i] + n * b0, d + i * n * nb, n * nb * sizeof(double));
+      });
 }
 
 }  // namespace
@@ -88,7 +96,7 @@ int cfft128_plan_create(size_t n, cfft_plan *out) {
   *out = nullptr;
   if (!is_pow2(n)) return set_error(CFFT_ERROR_NOT_POWER_OF_TWO, "assertion failed: n.is_power_of_two()");
   if (n < 32) return set_error(CFFT_ERROR_SIZE, "assertion failed: n >= 32");
-  if (ilog2(n) > 12) return set_error(CFFT_ERROR_SIZE, "fft128 device plans support n <= 4096");
+  if (ilog2(n) > 20) return set_error(CFFT_ERROR_SIZE, "fft128 device plans support n <= 2^20");
   auto *p = new (std::nothrow) cfft_plan_s();
   if (!p) return set_error(CFFT_ERROR_OUT_OF_MEMORY, "plan allocation failed");
   p->kind = cfft_plan_s::FFT128;

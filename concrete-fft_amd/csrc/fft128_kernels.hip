@@ -123,18 +123,21 @@ __device__ __forceinline__ uint32_t group_pos(uint32_t g, uint32_t ptop, uint32_
 }
 
 // One round of R stages on NG = 8 / R-point groups per thread.
-// FWD: stages st0, st0+1, .. (bit ptop = L-1-st0 downwards), CT butterflies.
+// FWD: stages st0, st0+1, .. (bit ptop = Lg-1-st0 downwards), CT butterflies.
 // INV: stages processed from the low bit up, GS butterflies.
+// Positions are local to a block of 2^L points that starts at point jhi of a
+// transform of 2^Lg points (jhi = 0, Lg = L for a whole transform); stage
+// indices and twiddles are those of the whole transform.
 template <int R, bool FWD, int MULV>
-__device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, uint32_t L,
-                                           const Tw &tw) {
+__device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, uint32_t Lg,
+                                           uint32_t jhi, const Tw &tw) {
   // z[u], u < 2^R, indexed with the bit for position ptop-b at bit (R-1-b)
   const uint32_t base = group_pos<R>(g, ptop, 0);
   if (FWD) {
 #pragma unroll
     for (int b = 0; b < R; ++b) {
       const uint32_t p = ptop - (uint32_t)b;       // pair bit
-      const uint32_t st = L - 1 - p;               // stage index, m = 2^st
+      const uint32_t st = Lg - 1 - p;              // stage index, m = 2^st
 #pragma unroll
       for (int u = 0; u < (1 << R); ++u) {
         if ((u >> (R - 1 - b)) & 1) continue;
@@ -142,7 +145,7 @@ __device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, ui
         uint32_t j = base;
 #pragma unroll
         for (int c = 0; c < R; ++c) j |= ((uint32_t)(u >> (R - 1 - c)) & 1u) << (ptop - c);
-        const uint32_t i = j >> (p + 1);
+        const uint32_t i = (jhi | j) >> (p + 1);
         const cdd w = load_tw(tw, (1u << st) + i);
         const cdd z1w = cmul<MULV>(z[v], w);
         const cdd z0 = z[u];
@@ -156,7 +159,7 @@ __device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, ui
 #pragma unroll
     for (int b = R - 1; b >= 0; --b) {
       const uint32_t p = ptop - (uint32_t)b;
-      const uint32_t st = L - 1 - p;
+      const uint32_t st = Lg - 1 - p;
 #pragma unroll
       for (int u = 0; u < (1 << R); ++u) {
         if ((u >> (R - 1 - b)) & 1) continue;
@@ -164,7 +167,7 @@ __device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, ui
         uint32_t j = base;
 #pragma unroll
         for (int c = 0; c < R; ++c) j |= ((uint32_t)(u >> (R - 1 - c)) & 1u) << (ptop - c);
-        const uint32_t i = j >> (p + 1);
+        const uint32_t i = (jhi | j) >> (p + 1);
         const cdd w = load_tw(tw, (1u << st) + i);
         const cdd z0 = z[u], z1 = z[v];
         cdd d;
@@ -180,8 +183,8 @@ __device__ __forceinline__ void round_regs(cdd *z, uint32_t g, uint32_t ptop, ui
 
 template <int R, bool FWD, int MULV>
 __device__ __forceinline__ void run_round(double *lds, const Bufs &gbuf, bool first, bool last,
-                                          uint32_t ptop, uint32_t L, uint32_t lt, uint32_t T,
-                                          const Tw &tw, bool active) {
+                                          uint32_t ptop, uint32_t L, uint32_t Lg, uint32_t jhi,
+                                          uint32_t lt, uint32_t T, const Tw &tw, bool active) {
   constexpr int P = 1 << R;
   constexpr int NG = 8 / P;
   const uint32_t n = 1u << L;
@@ -215,7 +218,7 @@ __device__ __forceinline__ void run_round(double *lds, const Bufs &gbuf, bool fi
 #pragma unroll
   for (int q = 0; q < NG; ++q) {
     const uint32_t g = lt + T * (uint32_t)q;
-    round_regs<R, FWD, MULV>(z[q], g, ptop, L, tw);
+    round_regs<R, FWD, MULV>(z[q], g, ptop, Lg, jhi, tw);
 #pragma unroll
     for (int u = 0; u < P; ++u) {
       const uint32_t j = group_pos<R>(g, ptop, (uint32_t)u);
@@ -238,20 +241,24 @@ __device__ __forceinline__ void run_round(double *lds, const Bufs &gbuf, bool fi
   __syncthreads();
 }
 
-// One workgroup per transform of n = 2^L points (L >= 3), T = n / 8 threads.
+// One workgroup per block of n = 2^L points (L >= 3), T = n / 8 threads:
+// the rounds over bits 0 .. L-1 of a transform of 2^Lg points (Lg >= L)
+// whose blocks are contiguous; workgroup b holds block b % 2^(Lg-L) of
+// transform b / 2^(Lg-L), at b * n.
 template <bool FWD, int MULV>
 __global__ void __launch_bounds__(512) k_fft128(double *re0, double *re1, double *im0,
                                                 double *im1, const double *__restrict__ t_re0,
                                                 const double *__restrict__ t_re1,
                                                 const double *__restrict__ t_im0,
                                                 const double *__restrict__ t_im1, uint32_t L,
-                                                uint64_t batch) {
+                                                uint32_t Lg, uint64_t blocks) {
   extern __shared__ double lds[];
   const uint32_t n = 1u << L;
   const uint32_t T = n / 8;
   const uint64_t tr = blockIdx.x;
-  const bool active = tr < batch;
+  const bool active = tr < blocks;
   const uint32_t lt = threadIdx.x;
+  const uint32_t jhi = (uint32_t)(tr & ((1ull << (Lg - L)) - 1)) << L;
   Bufs g{re0 + tr * n, re1 + tr * n, im0 + tr * n, im1 + tr * n};
   Tw tw{t_re0, t_re1, t_im0, t_im1};
   const uint32_t rounds = (L + 2) / 3;
@@ -270,10 +277,49 @@ __global__ void __launch_bounds__(512) k_fft128(double *re0, double *re1, double
     }
     const uint32_t ptop = lo_bit + nb - 1;
     switch (nb) {
-      case 3: run_round<3, FWD, MULV>(lds, g, first, last, ptop, L, lt, T, tw, active); break;
-      case 2: run_round<2, FWD, MULV>(lds, g, first, last, ptop, L, lt, T, tw, active); break;
-      default: run_round<1, FWD, MULV>(lds, g, first, last, ptop, L, lt, T, tw, active); break;
+      case 3: run_round<3, FWD, MULV>(lds, g, first, last, ptop, L, Lg, jhi, lt, T, tw, active); break;
+      case 2: run_round<2, FWD, MULV>(lds, g, first, last, ptop, L, Lg, jhi, lt, T, tw, active); break;
+      default: run_round<1, FWD, MULV>(lds, g, first, last, ptop, L, Lg, jhi, lt, T, tw, active); break;
     }
+  }
+}
+
+// One round of R stages (bits ptop .. ptop-R+1, all above the LDS block)
+// over global memory: a thread per R-bit group of 2^R points, loaded, run
+// through the round's butterflies in registers and stored in place.
+// Consecutive threads take consecutive groups, which sit at consecutive
+// points (the round's bits are all >= R), so every access is coalesced.
+template <int R, bool FWD, int MULV>
+__global__ void __launch_bounds__(256) k_fft128_global(double *re0, double *re1, double *im0,
+                                                       double *im1, const double *__restrict__ t_re0,
+                                                       const double *__restrict__ t_re1,
+                                                       const double *__restrict__ t_im0,
+                                                       const double *__restrict__ t_im1,
+                                                       uint32_t Lg, uint32_t ptop, uint64_t groups) {
+  constexpr int P = 1 << R;
+  const uint64_t gid = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (gid >= groups) return;
+  const uint64_t tr = gid >> (Lg - R);
+  const uint32_t g = (uint32_t)(gid & ((1ull << (Lg - R)) - 1));
+  const uint64_t off = tr << Lg;
+  Tw tw{t_re0, t_re1, t_im0, t_im1};
+  cdd z[P];
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    const uint64_t j = off + group_pos<R>(g, ptop, (uint32_t)u);
+    z[u].re.hi = re0[j];
+    z[u].re.lo = re1[j];
+    z[u].im.hi = im0[j];
+    z[u].im.lo = im1[j];
+  }
+  round_regs<R, FWD, MULV>(z, g, ptop, Lg, 0u, tw);
+#pragma unroll
+  for (int u = 0; u < P; ++u) {
+    const uint64_t j = off + group_pos<R>(g, ptop, (uint32_t)u);
+    re0[j] = z[u].re.hi;
+    re1[j] = z[u].re.lo;
+    im0[j] = z[u].im.hi;
+    im1[j] = z[u].im.lo;
   }
 }
 
@@ -428,32 +474,95 @@ hipError_t set_lds(const void *kernel, size_t bytes) {
 
 namespace cfft_launch {
 
-// Largest fft128 transform held in one workgroup: 4096 points = 136 KiB LDS.
-constexpr uint32_t kFft128MaxLog2 = 12;
+// Largest fft128 block held in one workgroup: 4096 points = 136 KiB LDS.
+// Larger transforms (up to 2^kFft128MaxLog2) run their stages above the
+// block over global memory: forward first (CT runs from the top bit down),
+// inverse last (GS runs from the bottom bit up).
+constexpr uint32_t kFft128BlockLog2 = 12;
+constexpr uint32_t kFft128MaxLog2 = 20;
+
+namespace {
+
+template <bool FWD, int MULV>
+hipError_t fft128_global_round(uint32_t R, uint32_t Lg, uint32_t ptop, double *re0, double *re1,
+                               double *im0, double *im1, const double *t_re0,
+                               const double *t_re1, const double *t_im0, const double *t_im1,
+                               uint64_t batch, hipStream_t s) {
+  const uint64_t groups = batch << (Lg - R);
+  const dim3 grid((uint32_t)((groups + 255) / 256)), block(256);
+  switch (R) {
+    case 3:
+      hipLaunchKernelGGL((k_fft128_global<3, FWD, MULV>), grid, block, 0, s, re0, re1, im0, im1,
+                         t_re0, t_re1, t_im0, t_im1, Lg, ptop, groups);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_fft128_global<2, FWD, MULV>), grid, block, 0, s, re0, re1, im0, im1,
+                         t_re0, t_re1, t_im0, t_im1, Lg, ptop, groups);
+      break;
+    default:
+      hipLaunchKernelGGL((k_fft128_global<1, FWD, MULV>), grid, block, 0, s, re0, re1, im0, im1,
+                         t_re0, t_re1, t_im0, t_im1, Lg, ptop, groups);
+      break;
+  }
+  return hipGetLastError();
+}
+
+template <bool FWD, int MULV>
+hipError_t fft128_blocks(uint32_t L, uint32_t Lg, double *re0, double *re1, double *im0,
+                         double *im1, const double *t_re0, const double *t_re1,
+                         const double *t_im0, const double *t_im1, uint64_t batch, hipStream_t s) {
+  const uint32_t n = 1u << L;
+  const uint64_t blocks = batch << (Lg - L);
+  const size_t lds = 4 * (size_t)pad128(n) * sizeof(double);
+  const void *k = reinterpret_cast<const void *>(k_fft128<FWD, MULV>);
+  hipError_t e = set_lds(k, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_fft128<FWD, MULV>), dim3((uint32_t)blocks), dim3(n / 8), lds, s, re0, re1,
+                     im0, im1, t_re0, t_re1, t_im0, t_im1, L, Lg, blocks);
+  return hipGetLastError();
+}
+
+template <bool FWD, int MULV>
+hipError_t fft128_any(uint32_t Lg, double *re0, double *re1, double *im0, double *im1,
+                      const double *t_re0, const double *t_re1, const double *t_im0,
+                      const double *t_im1, uint64_t batch, hipStream_t s) {
+  const uint32_t L = Lg < kFft128BlockLog2 ? Lg : kFft128BlockLog2;
+  hipError_t e = hipSuccess;
+  if (FWD) {
+    // bits Lg-1 .. L in rounds of up to 3 from the top, then the blocks
+    for (uint32_t top = Lg - 1; top + 1 > L && e == hipSuccess;) {
+      const uint32_t R = top + 1 - L >= 3 ? 3 : top + 1 - L;
+      e = fft128_global_round<FWD, MULV>(R, Lg, top, re0, re1, im0, im1, t_re0, t_re1, t_im0,
+                                         t_im1, batch, s);
+      top -= R;
+    }
+    if (e == hipSuccess)
+      e = fft128_blocks<FWD, MULV>(L, Lg, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s);
+  } else {
+    // the blocks, then bits L .. Lg-1 in rounds of up to 3 from the bottom
+    e = fft128_blocks<FWD, MULV>(L, Lg, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s);
+    for (uint32_t lo = L; lo < Lg && e == hipSuccess;) {
+      const uint32_t R = Lg - lo >= 3 ? 3 : Lg - lo;
+      e = fft128_global_round<FWD, MULV>(R, Lg, lo + R - 1, re0, re1, im0, im1, t_re0, t_re1,
+                                         t_im0, t_im1, batch, s);
+      lo += R;
+    }
+  }
+  return e;
+}
+
+}  // namespace
 
 hipError_t fft128_run(bool fwd, int mulv, uint32_t L, double *re0, double *re1, double *im0,
                       double *im1, const double *t_re0, const double *t_re1,
                       const double *t_im0, const double *t_im1, uint64_t batch, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   if (L < 3 || L > kFft128MaxLog2) return hipErrorInvalidValue;
-  const uint32_t n = 1u << L;
-  const size_t lds = 4 * (size_t)pad128(n) * sizeof(double);
-  const dim3 grid((uint32_t)batch), block(n / 8);
-  const void *k;
-  hipError_t e;
-#define CFFT128_LAUNCH(F, M)                                                                   \
-  k = reinterpret_cast<const void *>(k_fft128<F, M>);                                          \
-  e = set_lds(k, lds);                                                                         \
-  if (e != hipSuccess) return e;                                                               \
-  hipLaunchKernelGGL((k_fft128<F, M>), grid, block, lds, s, re0, re1, im0, im1, t_re0, t_re1, \
-                     t_im0, t_im1, L, batch);
-  if (fwd) {
-    if (mulv) { CFFT128_LAUNCH(true, 1) } else { CFFT128_LAUNCH(true, 0) }
-  } else {
-    if (mulv) { CFFT128_LAUNCH(false, 1) } else { CFFT128_LAUNCH(false, 0) }
-  }
-#undef CFFT128_LAUNCH
-  return hipGetLastError();
+  if (fwd)
+    return mulv ? fft128_any<true, 1>(L, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s)
+                : fft128_any<true, 0>(L, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s);
+  return mulv ? fft128_any<false, 1>(L, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s)
+              : fft128_any<false, 0>(L, re0, re1, im0, im1, t_re0, t_re1, t_im0, t_im1, batch, s);
 }
 
 // S: 16 * batch * n doubles; margin (batch doubles) may be null.  twiddles

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -201,27 +202,60 @@ hipError_t launch_schedule(const DevSchedule &d, uint32_t grid, bool r16_queue, 
   return e;
 }
 
-// best of 3 timed direct launches (after one warm-up) on the given buffers
+// best time of direct launches on the given buffers and stream, after one
+// warm-up: at least 3 timed launches, then more until `seconds` have passed
+// (at most 1000)
 hipError_t time_schedule(const DevSchedule &d, uint32_t grid, bool r16_queue, const double *x,
-                         dc64 *f, double *y, double *ms) {
+                         dc64 *f, double *y, hipStream_t s, double seconds, double *ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   double best = 1e30;
-  for (int it = 0; it < 4 && e == hipSuccess; ++it) {
-    e = hipEventRecord(e0, nullptr);
-    if (e == hipSuccess) e = launch_schedule(d, grid, r16_queue, x, f, y, nullptr);
-    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int it = 0; it < 1001 && e == hipSuccess; ++it) {
+    e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = launch_schedule(d, grid, r16_queue, x, f, y, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
     if (e == hipSuccess && it > 0) best = std::min(best, (double)t);
+    const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (it >= 3 && el >= seconds) break;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   *ms = best;
   return e;
 }
+
+// zero-filled stream-ordered timing buffers
+struct TuneBufs {
+  double *x = nullptr, *y = nullptr;
+  dc64 *f = nullptr;
+  hipError_t alloc(size_t nx, size_t nf, size_t ny, hipStream_t s) {
+    nx = std::max<size_t>(nx, 1);
+    ny = std::max<size_t>(ny, 1);
+    nf = std::max<size_t>(nf, 1);
+    hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&x), nx * sizeof(double), s);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void **>(&y), ny * sizeof(double), s);
+    if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void **>(&f), nf * sizeof(dc64), s);
+    if (e == hipSuccess) e = hipMemsetAsync(x, 0, nx * sizeof(double), s);
+    return e;
+  }
+  // frees in stream order and waits for the stream; the first failure wins
+  hipError_t free_on(hipStream_t s, hipError_t e) {
+    for (void *p : {(void *)x, (void *)y, (void *)f})
+      if (p) {
+        const hipError_t e2 = hipFreeAsync(p, s);
+        if (e == hipSuccess) e = e2;
+      }
+    x = y = nullptr;
+    f = nullptr;
+    const hipError_t e3 = hipStreamSynchronize(s);
+    return e == hipSuccess ? e3 : e;
+  }
+};
 
 struct Candidate {
   int algo;
@@ -283,7 +317,8 @@ struct cfft_uneven_s {
 namespace {
 
 // time one class alone with a candidate plan on packed synthetic buffers
-int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t grid, double *ms) {
+int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t grid,
+                   hipStream_t s, double seconds, double *ms) {
   cfft_plan p = nullptr;
   int rc = cfft_unordered_plan_create((size_t)1 << log2n, CFFT_METHOD_USER_PROVIDED, c.algo,
                                       c.base_n, 0.0, &p);
@@ -296,20 +331,15 @@ int time_candidate(uint32_t log2n, size_t njobs, const Candidate &c, uint32_t gr
   std::vector<UClass> cls{class_of(p)};
   HostSchedule h = make_schedule(by_class, cls);
   DevSchedule d;
-  double *x = nullptr, *y = nullptr;
-  dc64 *f = nullptr;
+  TuneBufs b;
   hipError_t e = upload_schedule(d, jobs, h, cls);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&x), 2 * n * njobs * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&y), 2 * n * njobs * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&f), n * njobs * sizeof(dc64));
-  if (e == hipSuccess) e = hipMemset(x, 0, 2 * n * njobs * sizeof(double));
+  if (e == hipSuccess) e = b.alloc(2 * n * njobs, n * njobs, 2 * n * njobs, s);
   double best = 1e30, t_static = 1e30;
-  if (e == hipSuccess) e = time_schedule(d, grid, true, x, f, y, &best);
-  if (e == hipSuccess && d.ntiles_r16) e = time_schedule(d, grid, false, x, f, y, &t_static);
+  if (e == hipSuccess) e = time_schedule(d, grid, true, b.x, b.f, b.y, s, seconds, &best);
+  if (e == hipSuccess && d.ntiles_r16)
+    e = time_schedule(d, grid, false, b.x, b.f, b.y, s, seconds, &t_static);
   best = std::min(best, t_static);
-  if (x) (void)hipFree(x);
-  if (y) (void)hipFree(y);
-  if (f) (void)hipFree(f);
+  e = b.free_on(s, e);
   d.release();
   cfft_plan_destroy(p);
   if (e != hipSuccess) return hip_error(e, "uneven autotuner");
@@ -334,28 +364,21 @@ int check_job(const cfft_uneven_job &j, size_t idx, size_t in_len, size_t f_len,
 }
 
 // Method::Measure: times the Dif16 launch with the device queue and with the
-// static deal on zeroed buffers of the schedule's own sizes (skipped, keeping
-// the queue, when they would take more than a quarter of free device memory)
-int choose_tile_schedule(cfft_uneven_s *u) {
+// static deal on zeroed buffers of the schedule's own sizes, for about
+// `seconds` each on the schedule's private stream (skipped, keeping the
+// queue, when they would take more than a quarter of free device memory)
+int choose_tile_schedule(cfft_uneven_s *u, double seconds) {
   size_t free_b = 0, total_b = 0;
   hipError_t e = hipMemGetInfo(&free_b, &total_b);
   if (e != hipSuccess) return hip_error(e, "device query");
   const size_t need = (u->in_len + u->out_len) * sizeof(double) + u->f_len * sizeof(dc64);
   if (need > free_b / 4) return CFFT_SUCCESS;
-  double *x = nullptr, *y = nullptr;
-  dc64 *f = nullptr;
-  e = hipMalloc(reinterpret_cast<void **>(&x), std::max<size_t>(u->in_len, 1) * sizeof(double));
-  if (e == hipSuccess)
-    e = hipMalloc(reinterpret_cast<void **>(&y), std::max<size_t>(u->out_len, 1) * sizeof(double));
-  if (e == hipSuccess)
-    e = hipMalloc(reinterpret_cast<void **>(&f), std::max<size_t>(u->f_len, 1) * sizeof(dc64));
-  if (e == hipSuccess) e = hipMemset(x, 0, std::max<size_t>(u->in_len, 1) * sizeof(double));
+  TuneBufs b;
+  e = b.alloc(u->in_len, u->f_len, u->out_len, u->cap);
   double t_queue = 0, t_static = 0;
-  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, true, x, f, y, &t_queue);
-  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, false, x, f, y, &t_static);
-  if (x) (void)hipFree(x);
-  if (y) (void)hipFree(y);
-  if (f) (void)hipFree(f);
+  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, true, b.x, b.f, b.y, u->cap, seconds, &t_queue);
+  if (e == hipSuccess) e = time_schedule(u->dev, u->grid, false, b.x, b.f, b.y, u->cap, seconds, &t_static);
+  e = b.free_on(u->cap, e);
   if (e != hipSuccess) return hip_error(e, "uneven autotuner");
   u->r16_queue = t_queue <= t_static;
   return CFFT_SUCCESS;
@@ -363,7 +386,7 @@ int choose_tile_schedule(cfft_uneven_s *u) {
 
 int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int method, int base_algo,
           double seconds) {
-  (void)seconds;
+  seconds = std::max(1e-3, seconds);
   std::vector<std::vector<uint32_t>> by_log2n(cfft_ub::kMaxLog2N + 1);
   u->jobs.resize(njobs);
   for (size_t i = 0; i < njobs; ++i) {
@@ -373,7 +396,10 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
   }
   int rc = grid_size(&u->grid);
   if (rc) return rc;
-  hipError_t ep = cfft_ub::prepare();
+  // the schedule's private stream: graph capture, and the autotuner's timing
+  hipError_t ep = hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking);
+  if (ep != hipSuccess) return hip_error(ep, "hipStreamCreateWithFlags");
+  ep = cfft_ub::prepare();
   if (ep != hipSuccess) return hip_error(ep, "uneven kernel attributes");
   std::vector<std::vector<uint32_t>> by_class;
   std::vector<UClass> classes;
@@ -388,7 +414,7 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
       double best = 1e30;
       for (const Candidate &c : candidates(l)) {
         double ms = 0;
-        rc = time_candidate(l, sample, c, u->grid, &ms);
+        rc = time_candidate(l, sample, c, u->grid, u->cap, seconds, &ms);
         if (rc) return rc;
         if (ms < best) {
           best = ms;
@@ -405,11 +431,10 @@ int build(cfft_uneven_s *u, const cfft_uneven_job *jobs, size_t njobs, int metho
     by_class.push_back(by_log2n[l]);
   }
   u->host = make_schedule(by_class, classes);
+  // synchronous copies: the schedule is on the device when this returns
   hipError_t e = upload_schedule(u->dev, u->jobs, u->host, classes);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&u->cap, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_error(e, "uneven schedule upload");
-  if (method == CFFT_METHOD_MEASURE && u->dev.ntiles_r16) return choose_tile_schedule(u);
+  if (method == CFFT_METHOD_MEASURE && u->dev.ntiles_r16) return choose_tile_schedule(u, seconds);
   return CFFT_SUCCESS;
 }
 

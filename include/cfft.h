@@ -67,7 +67,13 @@ typedef enum {
 
 /* ordered::Method (src/ordered.rs:48-59) / unordered::Method
  * (src/unordered.rs:526-537).  MEASURE is served by this library's GPU
- * autotuner (it times the candidate plans on the device). */
+ * autotuner: each candidate plan is timed for about measure_seconds on a
+ * private stream (no device-wide synchronisation) on the shapes it serves --
+ * the forward and inverse transform and, for unordered plans of
+ * n <= 8192, the fused negacyclic product -- and the choice is kept in the
+ * tuning cache (cfft_tune_cache_*), which later MEASURE plans of the same
+ * kind and size on the same device and stack reuse without timing.  HIP
+ * failures while timing are returned by the plan-creation call. */
 typedef enum { CFFT_METHOD_USER_PROVIDED = 0, CFFT_METHOD_MEASURE = 1 } cfft_method;
 
 /* ---- status / diagnostics ---- */
@@ -102,7 +108,8 @@ int cfft_event_elapsed_ms(cfft_event_t start, cfft_event_t stop, float *ms);
 /* ---- plans ---- */
 /* ordered::Plan::new(n, method), src/ordered.rs:242-278: n a power of two,
  * n <= 2^10.  `algo` is used for USER_PROVIDED; MEASURE times all 8 algos on
- * the device for about `measure_seconds` each (Method::Measure(Duration)). */
+ * the device for about `measure_seconds` each (Method::Measure(Duration)),
+ * or reuses the tuning cache's choice. */
 int cfft_ordered_plan_create(size_t n, int method, int algo, double measure_seconds,
                              cfft_plan *out);
 /* The same ordered plan for 2^11 <= n <= 2^16 (and any smaller n).  The
@@ -124,9 +131,41 @@ int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measur
 int cfft_unordered_plan_create(size_t n, int method, int base_algo, size_t base_n,
                                double measure_seconds, cfft_plan *out);
 /* fft128::Plan::new(n), src/fft128/mod.rs:1864-1885: n a power of two,
- * n >= 32. */
+ * n >= 32 (and n <= 2^20 on the device).  Up to 4096 points a transform
+ * runs in one workgroup's LDS; above, the stages above the 4096-point
+ * blocks run over global memory (before the blocks forward, after them
+ * inverse), with the same butterflies and so the same bits. */
 int cfft128_plan_create(size_t n, cfft_plan *out);
 int cfft_plan_destroy(cfft_plan plan);
+
+/* ---- tuning cache (no reference counterpart: the reference measures
+ *      again in every process) ----
+ * Entries map (device key, kind, n) to (algo, base_n); kind 0 = ordered,
+ * 1 = unordered.  The device key names the device model, ISA,
+ * compute-unit count, HIP runtime version and ABI version.  The default
+ * file is $CFFT_TUNE_CACHE, else $XDG_CACHE_HOME/cfft_amd/tune-cache.tsv,
+ * else $HOME/.cache/cfft_amd/tune-cache.tsv ("" or "off" in CFFT_TUNE_CACHE:
+ * no file).  It is read once, at the first MEASURE lookup, and rewritten
+ * (merged, through a rename) after each new measurement; a file that cannot
+ * be written leaves the choice in the process.  The file is a text table:
+ * one line per entry, device key, kind, n, algo and base_n separated by
+ * tabs; '#' lines are comments; lines that do not parse or name a choice
+ * no plan accepts are skipped. */
+/* The default file's path ("" when disabled); `len` gets its length. */
+int cfft_tune_cache_path(char *buf, size_t cap, size_t *len);
+/* The current device's key (needs a device). */
+int cfft_tune_device_key(char *buf, size_t cap, size_t *len);
+/* Adds the entries of `path` to the process table (they replace entries
+ * of the same key); `entries` gets the number of entries read. */
+int cfft_tune_cache_import(const char *path, size_t *entries);
+/* Writes the process table to `path`. */
+int cfft_tune_cache_export(const char *path);
+/* Empties the process table; the default file is not read again. */
+int cfft_tune_cache_clear(void);
+int cfft_tune_cache_size(size_t *entries);
+/* found = 1 and (algo, base_n) when the process table holds the key. */
+int cfft_tune_cache_lookup(const char *device_key, int kind, size_t n, int *found, int *algo,
+                           size_t *base_n);
 
 /* Plan::fft_size, src/ordered.rs:290 / src/unordered.rs:765 /
  * src/fft128/mod.rs:1896 */
@@ -190,8 +229,8 @@ int cfft128_inv_host(cfft_plan plan, double *re0, double *re1, double *im0, doub
                      size_t batch);
 /* #[doc(hidden)] free functions negacyclic_{fwd,inv}_fft{,_scalar,_avxfma}
  * (src/fft128/mod.rs:351-1102), with caller-provided twiddles (device
- * pointers, n each).  mul_variant 0 reproduces the scalar path's
- * double-double product, 1 the AVX2/AVX-512 path's fused form. */
+ * pointers, n each; 8 <= n <= 2^20).  mul_variant 0 reproduces the scalar
+ * path's double-double product, 1 the AVX2/AVX-512 path's fused form. */
 int cfft128_negacyclic_fwd_fft(size_t n, double *re0, double *re1, double *im0, double *im1,
                                const double *tw_re0, const double *tw_re1, const double *tw_im0,
                                const double *tw_im1, size_t batch, int mul_variant,
@@ -287,11 +326,13 @@ typedef struct cfft_uneven_s *cfft_uneven;
  * inside them).  method USER_PROVIDED: every size class uses base_algo with
  * base_n = min(n, 1024), and the Dif16 tiles run from the device queue;
  * MEASURE: each class uses the fastest of a candidate set of plans, timed on
- * the device (the choice replaces Method::Measure, src/unordered.rs:553-640,
- * for this workload), then the device queue and the static deal of the
- * Dif16 tiles are timed on the whole schedule (on temporary buffers of the
+ * the device for about measure_seconds each (the choice replaces
+ * Method::Measure, src/unordered.rs:553-640, for this workload), then the
+ * device queue and the static deal of the Dif16 tiles are timed on the whole
+ * schedule for about measure_seconds each (on temporary buffers of the
  * stated lengths, when they fit in a quarter of free device memory) and the
- * faster is kept.  Blocks until built. */
+ * faster is kept.  The timing runs on the schedule's private stream.  Blocks
+ * until built. */
 int cfft_uneven_create(const cfft_uneven_job *jobs, size_t njobs, size_t in_len,
                        size_t fourier_len, size_t out_len, int method, int base_algo,
                        double measure_seconds, cfft_uneven *out);

@@ -54,9 +54,9 @@ def test_ordered_rejects_too_large():
     assert "n.trailing_zeros() < 11" in str(e.value)
 
 
-@pytest.mark.parametrize("n,status", [(16384, 2), (3072, 1)])
+@pytest.mark.parametrize("n,status", [(131072, 2), (3072, 1)])
 def test_ordered_large_limits(n, status):
-    # the extended entry point stops where one workgroup's LDS ends (2^13)
+    # the extended entry point stops at the reference's documented 2^16
     import cfft_amd
     from cfft_amd.ordered import FftAlgo, Method, Plan
     with pytest.raises(cfft_amd.CfftPanic) as e:

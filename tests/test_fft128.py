@@ -151,10 +151,13 @@ def dev(arrs):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [8, 32, 64, 512, 1024, 2048, 4096])
+# n > 4096 runs the stages above the 4096-point LDS blocks over global
+# memory: 8192 (one round of 1 stage), 16384 (2), 32768 (3), 2^17 (3 + 2)
+@pytest.mark.parametrize("n", [8, 32, 64, 512, 1024, 2048, 4096, 8192, 16384, 32768, 131072])
 @pytest.mark.parametrize("mulv", [0, 1])
 def test_gpu_free_functions_bit_exact(cfft, O, n, mulv):
     import cfft_amd
+    print("fft128 free functions: n=%d mulv=%d" % (n, mulv), flush=True)
     rng = np.random.default_rng(n + mulv)
     batch = 3
     tw = O.fft128_twiddles(n)
@@ -176,7 +179,7 @@ def test_gpu_free_functions_bit_exact(cfft, O, n, mulv):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("N", [64, 256, 2048])
+@pytest.mark.parametrize("N", [64, 256, 2048, 16384, 32768])
 def test_gpu_plan_product_bound(cfft, N):
     # test_wrapper (src/fft128/mod.rs:1972-2065) through fft128::Plan on the device
     import cfft_amd
@@ -209,3 +212,5 @@ def test_gpu_plan_rejects_bad_sizes(cfft):
         cfft_amd.fft128.Plan(16)
     with pytest.raises(cfft_amd.CfftPanic):
         cfft_amd.fft128.Plan(100)
+    with pytest.raises(cfft_amd.CfftPanic):
+        cfft_amd.fft128.Plan(1 << 21)
