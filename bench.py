@@ -232,8 +232,14 @@ SERVE_SIZES = [1024, 2048, 4096]
 
 def bench_serve(duration_s, rates=SERVE_RATES):
     """Open-loop Poisson replay of external-product requests (k+1 = 2, l = 3,
-    base 2^8, N uniform over SERVE_SIZES) against resident Fourier keys."""
-    from cfft_amd.serve import Server
+    base 2^8, N uniform over SERVE_SIZES) against resident Fourier keys, in
+    two forms: host-buffer requests (inputs and results cross PCIe) and
+    device-buffer requests (CFFT_REQUEST_DEVICE_BUFFERS: inputs read from and
+    results written to device buffers, the form a CMux chain uses)."""
+    from cfft_amd import check, lib
+    from cfft_amd._lib import Request
+    from cfft_amd.device import DeviceBuffer
+    from cfft_amd.serve import DEVICE_BUFFERS, EXTERNAL_PRODUCT, Server
     rng = np.random.default_rng(5)
     srv = Server()
     keys = {}
@@ -241,49 +247,77 @@ def bench_serve(duration_s, rates=SERVE_RATES):
         key = rng.integers(0, 2**64, (6, 2, N), dtype=np.uint64)
         keys[N] = srv.add_key(key, 2, 3, 8)
     glwes = {N: rng.integers(0, 2**64, (64, 2, N), dtype=np.uint64) for N in SERVE_SIZES}
+    dglwes = {N: DeviceBuffer.from_host(glwes[N]) for N in SERVE_SIZES}
+    # device results go to an arena of ARENA slots used round-robin: request
+    # i and i + ARENA are 80 ms apart at 50k req/s, never in flight together
+    ARENA = 4096
+    slot_bytes = 2 * max(SERVE_SIZES) * 8
+    dout = DeviceBuffer((ARENA, slot_bytes // 8), np.uint64)
 
-    def trace(rate, count):
-        Ns = rng.choice(SERVE_SIZES, count)
-        made = [srv.extprod_request(keys[int(N)], glwes[int(N)][i % 64])
-                for i, N in enumerate(Ns)]
+    def trace(rate, count, dev):
+        Ns = [int(N) for N in rng.choice(SERVE_SIZES, count)]
+        if dev:
+            made = [(Request(EXTERNAL_PRODUCT | DEVICE_BUFFERS, keys[N], N,
+                             dglwes[N].dptr + (i % 64) * 2 * N * 8, None,
+                             dout.dptr + (i % ARENA) * slot_bytes), None)
+                    for i, N in enumerate(Ns)]
+        else:
+            made = [srv.extprod_request(keys[N], glwes[N][i % 64]) for i, N in enumerate(Ns)]
         arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
         return made, arrivals
 
-    # warm-up at every rate: the first passes of a process show one-time
-    # stalls of 10-20 ms (runtime resources growing with the number of
-    # batches in flight); steady-state serving is what is measured
-    for rate in rates:
-        made, arr = trace(rate, max(200, int(rate * 0.2)))
-        srv.replay([q for q, _ in made], arr)
-    out = []
-    for rate in rates:
-        count = max(200, int(rate * duration_s))
-        made, arr = trace(rate, count)
-        before = srv.stats()
-        lat = srv.replay([q for q, _ in made], arr)
-        after = srv.stats()
-        span = float(arr[-1] + lat[-1])
-        # results of the mix against the same request alone, on a sample
-        same = True
-        for i in rng.choice(count, 8, replace=False):
-            q, keep = srv.extprod_request(made[i][0].key_id,
-                                          glwes[int(made[i][0].N)][i % 64])
-            alone = srv.wait(srv.submit(q, keep))
-            same &= bool(np.array_equal(alone, made[i][1][-1]))
-        nb = after["batches"] - before["batches"]
-        out.append({"offered_req_per_s": rate, "requests": count,
-                    "achieved_req_per_s": round(count / span, 1),
-                    "p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
-                    "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
-                    "max_us": round(float(lat.max()) * 1e6, 1),
-                    "batches": nb, "mean_batch": round(count / max(nb, 1), 2),
-                    "max_inflight": after["max_inflight"],
-                    "alone_equals_mix": same})
+    def result(made, i, dev):
+        if not dev:
+            return made[i][1][-1]
+        N = int(made[i][0].N)
+        h = np.empty((2, N), np.uint64)
+        check(lib.cfft_memcpy_dtoh(h.ctypes.data, dout.dptr + (i % ARENA) * slot_bytes,
+                                   2 * N * 8, None))
+        return h
+
+    def run(dev):
+        # warm-up at every rate: the first passes of a process show one-time
+        # stalls of 10-20 ms (runtime resources growing with the number of
+        # batches in flight); steady-state serving is what is measured
+        for rate in rates:
+            made, arr = trace(rate, max(200, int(rate * 0.2)), dev)
+            srv.replay([q for q, _ in made], arr)
+        out = []
+        for rate in rates:
+            count = max(200, int(rate * duration_s))
+            made, arr = trace(rate, count, dev)
+            before = srv.stats()
+            lat = srv.replay([q for q, _ in made], arr)
+            after = srv.stats()
+            span = float(arr[-1] + lat[-1])
+            # results of the mix against the same request alone (host form),
+            # on a sample of the requests whose result slot is still theirs
+            same = True
+            for i in rng.choice(np.arange(max(0, count - ARENA), count), 8, replace=False):
+                i = int(i)
+                N = int(made[i][0].N)
+                q, keep = srv.extprod_request(keys[N], glwes[N][i % 64])
+                alone = srv.wait(srv.submit(q, keep))
+                same &= bool(np.array_equal(alone, result(made, i, dev)))
+            nb = after["batches"] - before["batches"]
+            out.append({"offered_req_per_s": rate, "requests": count,
+                        "achieved_req_per_s": round(count / span, 1),
+                        "p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
+                        "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1),
+                        "max_us": round(float(lat.max()) * 1e6, 1),
+                        "batches": nb, "mean_batch": round(count / max(nb, 1), 2),
+                        "max_inflight": after["max_inflight"],
+                        "alone_equals_mix": same})
+        return out
+
+    host = run(False)
+    device = run(True)
     st = srv.stats()
     srv.close()
     return {"workload": "external products, GLWE k+1=2 of N in %s, l=3, base 2^8, resident "
                         "Fourier keys, Poisson arrivals, %.2f s per rate" % (SERVE_SIZES, duration_s),
-            "rates": out, "hbm_budget_bytes": st["hbm_budget_bytes"], "streams": st["streams"]}
+            "rates": host, "device_buffer_rates": device,
+            "hbm_budget_bytes": st["hbm_budget_bytes"], "streams": st["streams"]}
 
 
 EXACT_SIZES = [2048, 4096]

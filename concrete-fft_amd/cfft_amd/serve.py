@@ -1,9 +1,11 @@
 """Request server: continuous batching of negacyclic products and external
 products on one device (cfft_server_* in include/cfft.h).
 
-Requests carry host numpy arrays; the server copies inputs when their batch
-launches and writes outputs when it completes, so the arrays must stay alive
-(this wrapper keeps references) until `wait` returns.
+Requests carry host numpy arrays (the server copies inputs when their batch
+launches and writes outputs when it completes) or, with the *_device
+request builders, DeviceBuffers of the server's device (gathered and
+scattered on the device); either way the buffers must stay alive (this
+wrapper keeps references) until `wait` returns.
 """
 import ctypes
 
@@ -13,6 +15,7 @@ from ._lib import Request, ServerConfig, ServerStatistics, check, lib
 
 POLYMUL = 0
 EXTERNAL_PRODUCT = 1
+DEVICE_BUFFERS = 0x100
 
 
 class Server:
@@ -49,6 +52,21 @@ class Server:
         assert glwe.shape == (k1, N)
         out = np.zeros_like(glwe)
         r = Request(EXTERNAL_PRODUCT, key_id, N, glwe.ctypes.data, None, out.ctypes.data)
+        return r, (glwe, out)
+
+    def polymul_request_device(self, a, b, out):
+        """a (N u64), b (N i64), out (N u64): DeviceBuffers; out is written
+        on the device."""
+        N = int(np.prod(a.shape))
+        r = Request(POLYMUL | DEVICE_BUFFERS, -1, N, a.dptr, b.dptr, out.dptr)
+        return r, (a, b, out)
+
+    def extprod_request_device(self, key_id, glwe, out):
+        """glwe, out: (k1, N) u64 DeviceBuffers (out may be glwe: the input
+        is gathered before the output is written)."""
+        N, k1 = self._keys[key_id]
+        assert int(np.prod(glwe.shape)) == k1 * N and int(np.prod(out.shape)) == k1 * N
+        r = Request(EXTERNAL_PRODUCT | DEVICE_BUFFERS, key_id, N, glwe.dptr, None, out.dptr)
         return r, (glwe, out)
 
     def submit(self, request, keep):

@@ -90,4 +90,13 @@ hipError_t r8_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, c
 hipError_t r8_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                       const dc64 *tw, const dc64 *twist, hipStream_t s);
 
+// Row copies between a contiguous block and caller buffers named by a table
+// of device addresses (`table`: `rows` u64 addresses, itself readable by the
+// device): gather copies row r from table[r] to dst + r * row_words, scatter
+// from src + r * row_words to table[r].  rows <= 65535.
+hipError_t gather_rows(const uint64_t *table, uint64_t rows, uint64_t row_words, uint64_t *dst,
+                       hipStream_t s);
+hipError_t scatter_rows(const uint64_t *table, uint64_t rows, uint64_t row_words,
+                        const uint64_t *src, hipStream_t s);
+
 }  // namespace cfft_launch

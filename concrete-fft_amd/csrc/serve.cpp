@@ -10,7 +10,10 @@
 //      requests -- the group (kind, N, key) of the oldest request, in arrival
 //      order, as many as fit the batch limit and the free device / staging
 //      memory -- and launches it: one host-to-device copy, one batched kernel,
-//      one device-to-host copy, one event;
+//      one device-to-host copy, one event (device-buffer requests: a gather
+//      kernel reads the inputs from the callers' device buffers, named by a
+//      table of addresses in mapped pinned staging, and a scatter kernel
+//      writes the results back; nothing crosses the bus but the table);
 //   3. sleeps until a submission arrives, or polls while batches are in flight.
 // So requests that arrive while others run on the device are admitted as soon
 // as a slot and memory free up, and batches on different streams overlap.
@@ -33,6 +36,7 @@
 #include <vector>
 
 #include "cfft.h"
+#include "launch.hpp"
 #include "plan.hpp"
 
 using cfft_internal::hip_error;
@@ -48,6 +52,8 @@ constexpr size_t kAlign = 256;
 constexpr int kIdleSpinUs = 500;
 constexpr int kMaxKeys = 256;
 constexpr int kMaxLog2N = 14;
+// rows of one gather / scatter launch (two input rows per product request)
+constexpr size_t kMaxDeviceBatch = 32767;
 
 size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -109,16 +115,19 @@ struct Key {
 struct Req {
   uint64_t ticket;
   int kind;
+  bool dev;      // in0, in1, out are device addresses
   int key;       // external product
   uint32_t log2N;
   const void *in0, *in1;
   void *out;
   size_t in_bytes, out_bytes;
+  size_t host_bytes;  // pinned staging: the data, or the address table
   Clock::time_point t_submit;
 };
 
 struct Batch {
   int kind, key;
+  bool dev;
   uint32_t log2N;
   std::vector<Req> reqs;
   size_t dev_off = 0, dev_len = 0, host_off = 0, host_len = 0, in_bytes = 0;
@@ -144,8 +153,9 @@ uint32_t ilog2(size_t n) {
 struct cfft_server_s {
   cfft_server_config cfg{};
   int device = 0;
-  // device and pinned staging pools, owned by the dispatcher
-  char *d_pool = nullptr, *h_pool = nullptr;
+  // device and pinned staging pools, owned by the dispatcher; h_pool_dev is
+  // the device's address of the (mapped) pinned pool
+  char *d_pool = nullptr, *h_pool = nullptr, *h_pool_dev = nullptr;
   RangeAlloc dalloc, halloc;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
@@ -246,18 +256,29 @@ hipError_t cfft_server_s::launch(Batch &b) {
   char *h_in = h_pool + b.host_off, *d_in = d_pool + b.dev_off;
   const size_t m = b.reqs.size();
   const size_t N = (size_t)1 << b.log2N;
-  // gather inputs into pinned staging
-  if (b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT) {
-    const size_t per = b.reqs[0].in_bytes;
-    for (size_t i = 0; i < m; ++i) memcpy(h_in + i * per, b.reqs[i].in0, per);
-  } else {
-    const size_t half = N * 8;
+  const size_t in_rows = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? m : 2 * m;
+  const size_t in_row_bytes = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? b.reqs[0].in_bytes : N * 8;
+  hipError_t e;
+  if (b.dev) {
+    // address table: input rows (products: all A rows, then all B rows),
+    // then output rows; the gather reads it over the bus
+    uint64_t *tab = reinterpret_cast<uint64_t *>(h_in);
     for (size_t i = 0; i < m; ++i) {
-      memcpy(h_in + i * half, b.reqs[i].in0, half);
-      memcpy(h_in + (m + i) * half, b.reqs[i].in1, half);
+      tab[i] = (uint64_t)(uintptr_t)b.reqs[i].in0;
+      if (b.kind == CFFT_REQUEST_POLYMUL) tab[m + i] = (uint64_t)(uintptr_t)b.reqs[i].in1;
+      tab[in_rows + i] = (uint64_t)(uintptr_t)b.reqs[i].out;
     }
+    e = cfft_launch::gather_rows(reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off),
+                                 in_rows, in_row_bytes / 8, reinterpret_cast<uint64_t *>(d_in), st);
+  } else {
+    // gather inputs into pinned staging
+    for (size_t i = 0; i < m; ++i) {
+      memcpy(h_in + i * in_row_bytes, b.reqs[i].in0, in_row_bytes);
+      if (b.kind == CFFT_REQUEST_POLYMUL)
+        memcpy(h_in + (m + i) * in_row_bytes, b.reqs[i].in1, in_row_bytes);
+    }
+    e = hipMemcpyAsync(d_in, h_in, b.in_bytes, hipMemcpyHostToDevice, st);
   }
-  hipError_t e = hipMemcpyAsync(d_in, h_in, b.in_bytes, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return e;
   char *d_out = d_in + round_up(b.in_bytes);
   int rc;
@@ -276,15 +297,21 @@ hipError_t cfft_server_s::launch(Batch &b) {
                                      reinterpret_cast<uint64_t *>(d_out), m, st);
   }
   if (rc != CFFT_SUCCESS) return hipErrorLaunchFailure;
-  const size_t out_bytes = m * b.reqs[0].out_bytes;
-  e = hipMemcpyAsync(h_pool + b.host_off + round_up(b.in_bytes), d_out, out_bytes,
-                     hipMemcpyDeviceToHost, st);
+  const size_t per_out = b.reqs[0].out_bytes;
+  if (b.dev) {
+    e = cfft_launch::scatter_rows(
+        reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off) + in_rows, m, per_out / 8,
+        reinterpret_cast<const uint64_t *>(d_out), st);
+  } else {
+    e = hipMemcpyAsync(h_pool + b.host_off + round_up(b.in_bytes), d_out, m * per_out,
+                       hipMemcpyDeviceToHost, st);
+  }
   if (e != hipSuccess) return e;
   return hipEventRecord(events[b.slot], st);
 }
 
 void cfft_server_s::retire(Batch &b, hipError_t err) {
-  if (err == hipSuccess) {
+  if (err == hipSuccess && !b.dev) {
     const char *h_out = h_pool + b.host_off + round_up(b.in_bytes);
     const size_t per = b.reqs[0].out_bytes;
     for (size_t i = 0; i < b.reqs.size(); ++i) memcpy(b.reqs[i].out, h_out + i * per, per);
@@ -324,32 +351,42 @@ bool cfft_server_s::launch_one() {
   Batch b;
   b.kind = head.kind;
   b.key = head.key;
+  b.dev = head.dev;
   b.log2N = head.log2N;
   b.slot = slot;
+  const size_t limit = b.dev ? std::min<size_t>(cfg.max_batch, kMaxDeviceBatch) : cfg.max_batch;
+  // pinned staging of a batch: its data (host requests, inputs then
+  // outputs) or its address table (device requests)
+  auto host_need = [&](size_t in_sum, size_t out_sum, size_t tab_sum) {
+    return b.dev ? round_up(tab_sum) : round_up(in_sum) + round_up(out_sum);
+  };
   // collect the head's group in arrival order while memory allows
-  size_t in_sum = 0, out_sum = 0;
+  size_t in_sum = 0, out_sum = 0, tab_sum = 0;
   std::vector<size_t> take;
-  for (size_t i = 0; i < waiting.size() && take.size() < cfg.max_batch; ++i) {
+  for (size_t i = 0; i < waiting.size() && take.size() < limit; ++i) {
     const Req &r = waiting[i];
-    if (r.kind != b.kind || r.log2N != b.log2N || r.key != b.key) continue;
+    if (r.kind != b.kind || r.dev != b.dev || r.log2N != b.log2N || r.key != b.key) continue;
     const size_t need = round_up(in_sum + r.in_bytes) + round_up(out_sum + r.out_bytes);
     const size_t dneed = need + dev_scratch(b, take.size() + 1);
-    if (dneed > dalloc.capacity() || need > halloc.capacity()) break;
+    const size_t hneed = host_need(in_sum + r.in_bytes, out_sum + r.out_bytes,
+                                   tab_sum + r.host_bytes);
+    if (dneed > dalloc.capacity() || hneed > halloc.capacity()) break;
     take.push_back(i);
     in_sum += r.in_bytes;
     out_sum += r.out_bytes;
+    tab_sum += r.host_bytes;
   }
   // shrink until both pools have room (admission by free memory)
   while (!take.empty()) {
-    const size_t need = round_up(in_sum) + round_up(out_sum);
-    const size_t dneed = need + dev_scratch(b, take.size());
+    const size_t dneed = round_up(in_sum) + round_up(out_sum) + dev_scratch(b, take.size());
+    const size_t hneed = host_need(in_sum, out_sum, tab_sum);
     size_t doff = 0, hoff = 0;
     if (dalloc.alloc(dneed, &doff)) {
-      if (halloc.alloc(need, &hoff)) {
+      if (halloc.alloc(hneed, &hoff)) {
         b.dev_off = doff;
         b.host_off = hoff;
         b.dev_len = dneed;
-        b.host_len = need;
+        b.host_len = hneed;
         break;
       }
       dalloc.release(doff, dneed);
@@ -357,6 +394,7 @@ bool cfft_server_s::launch_one() {
     const Req &last = waiting[take.back()];
     in_sum -= last.in_bytes;
     out_sum -= last.out_bytes;
+    tab_sum -= last.host_bytes;
     take.pop_back();
   }
   if (take.empty()) {
@@ -442,7 +480,10 @@ int cfft_server_create(const cfft_server_config *config, cfft_server *out) {
   s->cfg = c;
   (void)hipGetDevice(&s->device);
   e = hipMalloc(reinterpret_cast<void **>(&s->d_pool), c.hbm_budget_bytes);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s->h_pool), c.pinned_bytes, 0);
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void **>(&s->h_pool), c.pinned_bytes, hipHostMallocMapped);
+  if (e == hipSuccess)
+    e = hipHostGetDevicePointer(reinterpret_cast<void **>(&s->h_pool_dev), s->h_pool, 0);
   for (uint32_t i = 0; i < c.streams && e == hipSuccess; ++i) {
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
@@ -523,23 +564,47 @@ int cfft_server_add_key(cfft_server s, size_t N, int k1, int levels, int base_lo
   return CFFT_SUCCESS;
 }
 
+}  // extern "C"
+
+namespace {
+
+// a device-buffer request's buffer: 8-byte aligned device (or managed)
+// memory of the server's device
+int check_device_buffer(const cfft_server_s *s, const void *p) {
+  if ((uintptr_t)p % 8) return set_error(CFFT_ERROR_INVALID_ARGUMENT, "request buffer is not 8-byte aligned");
+  hipPointerAttribute_t a{};
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "request buffer is not device memory");
+  }
+  if ((a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged) || a.device != s->device)
+    return set_error(CFFT_ERROR_INVALID_ARGUMENT, "request buffer is not memory of the server's device");
+  return CFFT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
 int cfft_server_submit(cfft_server s, const cfft_request *r, uint64_t *ticket) {
   if (!s || !r || !ticket) return set_error(CFFT_ERROR_NULL_POINTER, "null argument");
   Req q{};
-  q.kind = r->kind;
+  q.dev = (r->kind & CFFT_REQUEST_DEVICE_BUFFERS) != 0;
+  q.kind = r->kind & ~CFFT_REQUEST_DEVICE_BUFFERS;
   q.in0 = r->in0;
   q.in1 = r->in1;
   q.out = r->out;
   q.key = -1;
   const Key *k = nullptr;
-  if (r->kind == CFFT_REQUEST_EXTERNAL_PRODUCT) {
+  if (q.kind == CFFT_REQUEST_EXTERNAL_PRODUCT) {
     if (r->key_id < 0 || r->key_id >= s->nkeys.load(std::memory_order_acquire))
       return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown key id");
     k = &s->keys[r->key_id];
     q.key = r->key_id;
     q.log2N = ilog2(k->N);
     if (!r->in0 || !r->out) return set_error(CFFT_ERROR_NULL_POINTER, "null request buffer");
-  } else if (r->kind == CFFT_REQUEST_POLYMUL) {
+  } else if (q.kind == CFFT_REQUEST_POLYMUL) {
     if (r->N < 4 || (r->N & (r->N - 1)) || ilog2(r->N) > kMaxLog2N)
       return set_error(CFFT_ERROR_SIZE, "polynomial size must be a power of two in [4, 16384]");
     if (!r->in0 || !r->in1 || !r->out) return set_error(CFFT_ERROR_NULL_POINTER, "null request buffer");
@@ -550,11 +615,20 @@ int cfft_server_submit(cfft_server s, const cfft_request *r, uint64_t *ticket) {
   } else {
     return set_error(CFFT_ERROR_INVALID_ARGUMENT, "unknown request kind");
   }
+  if (q.dev) {
+    for (const void *p : {q.in0, q.kind == CFFT_REQUEST_POLYMUL ? q.in1 : q.in0, (const void *)q.out}) {
+      const int rc = check_device_buffer(s, p);
+      if (rc) return rc;
+    }
+  }
   q.in_bytes = req_in_bytes(q.kind, k, q.log2N);
   q.out_bytes = req_out_bytes(q.kind, k, q.log2N);
+  q.host_bytes = q.dev ? (q.kind == CFFT_REQUEST_POLYMUL ? 3 : 2) * sizeof(uint64_t)
+                       : q.in_bytes + q.out_bytes;
   const size_t need = round_up(q.in_bytes) + round_up(q.out_bytes);
   const size_t dneed = need + (k ? key_scratch(*k, 1) : 0);
-  if (dneed > s->cfg.hbm_budget_bytes || need > s->cfg.pinned_bytes)
+  const size_t hneed = q.dev ? round_up(q.host_bytes) : need;
+  if (dneed > s->cfg.hbm_budget_bytes || hneed > s->cfg.pinned_bytes)
     return set_error(CFFT_ERROR_BUSY, "request larger than the server's memory budget");
   {
     std::lock_guard<std::mutex> g(s->mu);

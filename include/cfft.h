@@ -383,7 +383,18 @@ typedef struct {
   uint32_t streams;        /* launch slots, 0: 4 (at most 16) */
 } cfft_server_config;
 typedef enum { CFFT_REQUEST_POLYMUL = 0, CFFT_REQUEST_EXTERNAL_PRODUCT = 1 } cfft_request_kind;
-/* Host buffers, owned by the caller and valid until the request completes.
+/* OR-ed into cfft_request.kind: in0, in1 and out are device buffers of the
+ * server's device (8-byte aligned; checked at submission).  The batch's
+ * inputs are gathered from them and its results scattered back by kernels
+ * on the device, so nothing but a table of addresses crosses the bus: a
+ * chain of requests on one device accumulator (a CMux chain of blind
+ * rotation) stays on the device.  The inputs must be complete when the
+ * request is submitted (the server does not wait on the caller's streams);
+ * the output is complete when the ticket is done.  Device requests batch
+ * with device requests of the same kind, N and key, up to 32767 a batch. */
+#define CFFT_REQUEST_DEVICE_BUFFERS 0x100
+/* Host buffers (or device buffers, with CFFT_REQUEST_DEVICE_BUFFERS), owned
+ * by the caller and valid until the request completes.
  * POLYMUL: in0 = A (N u64 torus), in1 = B (N i64), out = N u64 (as
  * cfft_negacyclic_polymul_u64 with a Dif16 plan, base_n = min(N/2, 1024)).
  * EXTERNAL_PRODUCT: key_id from cfft_server_add_key, in0 = GLWE (k1 x N

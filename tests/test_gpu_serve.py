@@ -253,3 +253,91 @@ def test_two_waiters_one_ticket(cfft):
             x.join(30)
         assert sorted(r[0] for r in res) == ["err", "ok"]
     srv.close()
+
+
+def _make_device(server, keys, r):
+    from cfft_amd.device import DeviceBuffer
+    kind, N, data = r
+    if kind == "polymul":
+        a, b = (DeviceBuffer.from_host(x) for x in data)
+        out = DeviceBuffer((N,), np.uint64)
+        return server.polymul_request_device(a, b, out)
+    g = DeviceBuffer.from_host(data)
+    out = DeviceBuffer((K1, N), np.uint64)
+    return server.extprod_request_device(keys[N][0], g, out)
+
+
+def test_device_requests_alone_equals_mix_equals_host(cfft):
+    """Device-buffer requests: alone, in a burst mixed with host-buffer
+    requests of the same shapes, and against the host-buffer results (which
+    equal the direct entry points, test above), bit for bit."""
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(15)
+    srv = Server()
+    keys = _setup(srv, rng)
+    reqs = _requests(srv, keys, rng, 48)
+    host = []
+    for r in reqs:
+        q, keep = _make(srv, keys, r)
+        host.append(srv.wait(srv.submit(q, keep)).copy())
+    alone = []
+    for r in reqs:
+        q, keep = _make_device(srv, keys, r)
+        alone.append(srv.wait(srv.submit(q, keep)).download().reshape(host[len(alone)].shape))
+    tickets = []
+    for r in reqs:  # device and host requests interleaved
+        for make in (_make_device, _make):
+            q, keep = make(srv, keys, r)
+            tickets.append((make is _make_device, srv.submit(q, keep)))
+    for i, (dev, t) in enumerate(tickets):
+        res = srv.wait(t)
+        res = res.download().reshape(host[i // 2].shape) if dev else res
+        assert np.array_equal(res, host[i // 2]), (i, dev)
+    for i in range(len(reqs)):
+        assert np.array_equal(alone[i], host[i]), i
+    srv.close()
+
+
+def test_device_chain_in_place(cfft):
+    """A CMux-style chain: five external products on one device accumulator,
+    each request reading and writing the same buffer, equal to five direct
+    external products."""
+    from cfft_amd import products
+    from cfft_amd.device import DeviceBuffer
+    from cfft_amd.serve import Server
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(16)
+    N = 2048
+    srv = Server()
+    keys = _setup(srv, rng, sizes=(N,))
+    kid, key = keys[N]
+    glwe = rng.integers(0, M, (K1, N), dtype=np.uint64)
+    acc = DeviceBuffer.from_host(glwe)
+    for _ in range(5):
+        q, keep = srv.extprod_request_device(kid, acc, acc)
+        srv.wait(srv.submit(q, keep))
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    kf = DeviceBuffer((K1 * LEVELS, K1, N // 2), np.complex128)
+    products.ggsw_to_fourier(plan, DeviceBuffer.from_host(key), K1, LEVELS, kf)
+    cur = DeviceBuffer.from_host(glwe[None])
+    for _ in range(5):
+        nxt = DeviceBuffer((1, K1, N), np.uint64)
+        products.external_product(plan, kf, K1, LEVELS, BASE_LOG, cur, nxt, 1)
+        cur = nxt
+    assert np.array_equal(acc.download().reshape(K1, N), cur.download()[0])
+    srv.close()
+
+
+def test_device_request_refuses_host_memory(cfft):
+    from cfft_amd import CfftPanic
+    from cfft_amd._lib import Request
+    from cfft_amd.serve import DEVICE_BUFFERS, POLYMUL, Server
+    srv = Server()
+    a = np.zeros(1024, np.uint64)
+    b = np.zeros(1024, np.int64)
+    out = np.zeros(1024, np.uint64)
+    r = Request(POLYMUL | DEVICE_BUFFERS, -1, 1024, a.ctypes.data, b.ctypes.data, out.ctypes.data)
+    with pytest.raises(CfftPanic) as ei:
+        srv.submit(r, (a, b, out))
+    assert ei.value.status == 10  # CFFT_ERROR_INVALID_ARGUMENT
+    srv.close()
