@@ -279,19 +279,65 @@ __device__ __forceinline__ void run_group_r16_local(const UClass &C, const UTile
   }
 }
 
-// n = 2^LOG2N = R * 1024 (R = 2, 4, 8): one job on R wavefronts.  Depth
+// n = 2^LOG2N = R * 1024 (R = 4, 8): one job on R wavefronts.  Depth
 // pass (DEPTH_FWD, radix R over the whole job; butterfly j < 1024 reads the
 // points k * 1024 + j, its output k, times T[k j], goes to chunk bitrev(k)),
 // then wave c runs the base transform of chunk c, forward and inverse, then
 // the inverse depth pass (DEPTH_INV: conj T[k j] before the butterfly).
 // Chunk c lives in LDS at c * 1088 points (r16 padding); the base transform
 // uses that same region as its exchange buffer.
+//
+// A thread's depth-pass inputs are 16 points (NBD butterflies of R points):
+// point (i, k) at p = k * 1024 + t + TJ * i.  The first kAhead of them can be
+// held raw (x_p, x_{p+n}) in a DepthIn, loaded a tile ahead; the rest are
+// loaded when the tile starts (holding all 16 across the base transforms
+// would spill).
+constexpr int kAhead = 8;
+struct DepthIn {
+  double re[kAhead], im[kAhead];
+};
+
 template <int LOG2N>
+__device__ __forceinline__ void depth_load(const UTile &tile, const UJob *__restrict__ jobs,
+                                           const uint32_t *__restrict__ order, const double *in,
+                                           uint32_t tid, DepthIn &d) {
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr int R = 1 << (LOG2N - 10);
+  constexpr uint32_t TJ = n / 16, NBD = 16 / R;
+  const uint32_t g = tid / TJ, t = tid % TJ;
+  // a group past the tile's last job reads the tile's first job (no branch
+  // around the loads: every load is issued, none is waited on here); its
+  // results are never stored
+  const double *x = in + jobs[order[tile.first + (g < tile.count ? g : 0u)]].in_off;
+#pragma unroll
+  for (int q = 0; q < kAhead; ++q) {
+    const uint32_t p = (uint32_t)(q % R) * 1024 + t + TJ * (uint32_t)(q / R);
+    d.re[q] = ld_stream(&x[p]);
+    d.im[q] = ld_stream(&x[p + n]);
+  }
+  (void)NBD;
+}
+
+__device__ __forceinline__ void depth_load_any(uint32_t log2n, const UTile &tile,
+                                               const UJob *__restrict__ jobs,
+                                               const uint32_t *__restrict__ order,
+                                               const double *in, uint32_t tid, DepthIn &d) {
+  if (log2n == 12) depth_load<12>(tile, jobs, order, in, tid, d);
+  else depth_load<13>(tile, jobs, order, in, tid, d);
+}
+
+// `after_barrier()` runs right after the first barrier: `din` is free from
+// there on (the persistent kernel loads the next tile's inputs into it,
+// overlapping the base transforms and the inverse depth pass).
+// AHEAD: the first kAhead points are in `din` (else every point is loaded
+// here).
+template <int LOG2N, bool AHEAD, class AfterBarrier>
 __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &tile,
                                                const UJob *__restrict__ jobs,
                                                const uint32_t *__restrict__ order,
                                                const double *in, c64 *fourier, double *out,
-                                               c64 *smem, const c64 *twl, uint32_t tid) {
+                                               c64 *smem, const c64 *twl, uint32_t tid,
+                                               DepthIn &din, AfterBarrier after_barrier) {
   using S = R16<10>;
   constexpr uint32_t n = 1u << LOG2N;
   constexpr int R = 1 << (LOG2N - 10);
@@ -314,8 +360,12 @@ __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &til
 #pragma unroll
       for (int k = 0; k < R; ++k) {
         const uint32_t p = (uint32_t)k * 1024 + j;
-        v[i][k] = active ? cmul(twist[p], mk(ld_stream(&P.x[p]), ld_stream(&P.x[p + n])))
-                         : mk(0.0, 0.0);
+        const int q = (int)i * R + k;
+        if (AHEAD && q < kAhead)
+          v[i][k] = cmul(twist[p], mk(din.re[q < kAhead ? q : 0], din.im[q < kAhead ? q : 0]));
+        else
+          v[i][k] = active ? cmul(twist[p], mk(ld_stream(&P.x[p]), ld_stream(&P.x[p + n])))
+                           : mk(0.0, 0.0);
       }
     }
 #pragma unroll
@@ -329,6 +379,7 @@ __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &til
     }
   }
   __syncthreads();
+  after_barrier();
   {
     const uint32_t c = t >> 6, l = t & 63;
     c64 *xc = X + c * CH;
@@ -348,45 +399,48 @@ __device__ __forceinline__ void tile_r16_depth(const UClass &C, const UTile &til
     for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
   }
   __syncthreads();
-  {
-    c64 v[NBD][R];
+  // one butterfly at a time: its R points, R - 1 twiddles and R untwist
+  // factors are all that is live beside the inputs loaded ahead
+#pragma unroll 1
+  for (uint32_t i = 0; i < NBD; ++i) {
+    const uint32_t j = t + TJ * i;
+    c64 v[R];
 #pragma unroll
-    for (uint32_t i = 0; i < NBD; ++i) {
-      const uint32_t j = t + TJ * i;
+    for (int k = 0; k < R; ++k) v[k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
 #pragma unroll
-      for (int k = 0; k < R; ++k) v[i][k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
+    for (int k = 1; k < R; ++k) {
+      c64 w = tw[(uint32_t)k * j];
+      w.im = -w.im;
+      v[k] = cmul(w, v[k]);
     }
+    butterfly<R, false>(v);
+    if (active) {
 #pragma unroll
-    for (uint32_t i = 0; i < NBD; ++i) {
-      const uint32_t j = t + TJ * i;
-#pragma unroll
-      for (int k = 1; k < R; ++k) {
-        c64 w = tw[(uint32_t)k * j];
-        w.im = -w.im;
-        v[i][k] = cmul(w, v[i][k]);
-      }
-      butterfly<R, false>(v[i]);
-      if (active) {
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-          const uint32_t p = (uint32_t)k * 1024 + j;
-          const c64 r = cmul(untwist[p], v[i][k]);
-          st_stream(r.re, &P.y[p]);
-          st_stream(r.im, &P.y[p + n]);
-        }
+      for (int k = 0; k < R; ++k) {
+        const uint32_t p = (uint32_t)k * 1024 + j;
+        const c64 r = cmul(untwist[p], v[k]);
+        st_stream(r.re, &P.y[p]);
+        st_stream(r.im, &P.y[p + n]);
       }
     }
   }
 }
 
+// a depth tile whose inputs were not loaded ahead
 __device__ __forceinline__ void run_tile_r16_depth(const UClass &C, const UTile &tile,
                                                    const UJob *__restrict__ jobs,
                                                    const uint32_t *__restrict__ order,
                                                    const double *in, c64 *fourier, double *out,
                                                    c64 *smem, const c64 *twl, uint32_t tid) {
+  DepthIn din;
+  auto none = [] {};
   switch (C.log2n) {
-    case 12: tile_r16_depth<12>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
-    default: tile_r16_depth<13>(C, tile, jobs, order, in, fourier, out, smem, twl, tid); break;
+    case 12:
+      tile_r16_depth<12, false>(C, tile, jobs, order, in, fourier, out, smem, twl, tid, din, none);
+      break;
+    default:
+      tile_r16_depth<13, false>(C, tile, jobs, order, in, fourier, out, smem, twl, tid, din, none);
+      break;
   }
 }
 
@@ -405,6 +459,13 @@ __device__ __forceinline__ void stage_r16_twiddles(const c64 *__restrict__ src, 
 // Without `counter` the tiles are dealt round-robin (workgroup w: tiles w,
 // w + G, ...).  Device-scope atomics on one address are serialised (measured
 // ~50 ns each); the partitions divide that cost, the static deal removes it.
+//
+// The schedule queues the depth classes (n = 4096, 8192) first.  While a
+// workgroup's tiles are depth tiles, each tile's inputs are loaded into
+// registers during the previous tile, right after its first barrier (the
+// next index is published before that barrier), so the loads overlap the
+// base transforms.  The remaining tiles run as before; a depth tile met
+// there loads its own inputs.
 __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
     const UJob *__restrict__ jobs, const uint32_t *__restrict__ order,
     const UTile *__restrict__ tiles, uint32_t ntiles, const UClass *__restrict__ classes,
@@ -423,8 +484,49 @@ __global__ void __launch_bounds__(kThreads) k_uneven_r16_persistent(
   if (queue && threadIdx.x == 0) s_next[0] = part + P * atomicAdd(ctr, 1u);
   __syncthreads();
   uint32_t t = queue ? __builtin_amdgcn_readfirstlane(s_next[0]) : blockIdx.x;
+  uint32_t slot = 1;
+  // depth tiles, inputs loaded a tile ahead
+  if (t < ntiles && classes[tiles[t].cls].log2n >= 12) {
+    DepthIn din;
+    {
+      const UTile tile = tiles[t];
+      depth_load_any(classes[tile.cls].log2n, tile, jobs, order, in,
+                     launder_u32(threadIdx.x), din);
+    }
+    bool ahead = true;
+    while (ahead) {
+      uint32_t next = 0;
+      if (queue && threadIdx.x == 0) next = part + P * atomicAdd(ctr, 1u);
+      const UTile tile = tiles[t];
+      const UClass &C = classes[tile.cls];
+      const uint32_t tid = launder_u32(threadIdx.x);
+      uint32_t tn = 0;
+      if (queue && threadIdx.x == 0) s_next[slot] = next;
+      auto load_next = [&] {
+        tn = queue ? __builtin_amdgcn_readfirstlane(s_next[slot]) : t + gridDim.x;
+        ahead = false;
+        if (tn < ntiles) {
+          const UTile nt = tiles[tn];
+          const uint32_t nl = classes[nt.cls].log2n;
+          if (nl >= 12) {
+            depth_load_any(nl, nt, jobs, order, in, launder_u32(threadIdx.x), din);
+            ahead = true;
+          }
+        }
+      };
+      if (C.log2n == 12)
+        tile_r16_depth<12, true>(C, tile, jobs, order, in, fourier, out, smem, twl, tid, din,
+                                 load_next);
+      else
+        tile_r16_depth<13, true>(C, tile, jobs, order, in, fourier, out, smem, twl, tid, din,
+                                 load_next);
+      __syncthreads();
+      t = tn;
+      slot ^= 1u;
+    }
+  }
   // queue slots: a slot is written one tile after every thread read it
-  for (uint32_t slot = 1; t < ntiles; slot ^= 1u) {
+  for (; t < ntiles; slot ^= 1u) {
     uint32_t next = 0;
     if (queue && threadIdx.x == 0) next = part + P * atomicAdd(ctr, 1u);
     const UTile tile = tiles[t];
