@@ -62,6 +62,7 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_tw_r16) (void)hipFree(d_tw_r16);
   if (d_tw_base1024) (void)hipFree(d_tw_base1024);
   if (d_tw_r8) (void)hipFree(d_tw_r8);
+  if (d_tw_chunk) (void)hipFree(d_tw_chunk);
   if (d_untwist) (void)hipFree(d_untwist);
   if (d_tw128) (void)hipFree(d_tw128);
 }
@@ -228,9 +229,24 @@ int build_f64_plan(cfft_plan_s *p) {
                          hipMemcpyHostToDevice));
     }
   }
+  // Dif16, base_n = 1024, n > 8192: the global depth passes end at a chunk
+  // of 2048, 4096 or 8192 points (the last depth level of radix 2, 4 or 8
+  // above the base, src/unordered.rs:391-489), which the wave-local kernels
+  // run with that level's twiddles
+  const bool chunked = p->algo == CFFT_DIF16 && p->base_n == 1024 && n > 8192 &&
+                       p->chunk_log2_fwd == p->chunk_log2_inv && p->chunk_log2_fwd >= 11 &&
+                       p->chunk_log2_fwd <= 13;
+  if (chunked) {
+    const size_t c = (size_t)1 << p->chunk_log2_fwd, mul = n / c;
+    std::vector<cfft_host::C64> tc(c);
+    for (size_t i = 0; i < c; ++i) tc[i] = tw[i * mul];
+    CFFT_HIP(hipMalloc(&p->d_tw_chunk, c * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(p->d_tw_chunk, tc.data(), c * sizeof(dc64), hipMemcpyHostToDevice));
+  }
   // the 1024-point r16 block of the plan's table, for the wave-local forms
-  // with a radix-2 depth pass (n = 2 base_n) and the uneven / product kernels
-  if (p->algo == CFFT_DIF16 && ((p->base_n == 1024 && n > 1024 && n <= 8192) ||
+  // with a radix-2 depth pass (n = 2 base_n), the chunks above and the
+  // uneven / product kernels
+  if (p->algo == CFFT_DIF16 && ((p->base_n == 1024 && n > 1024 && (n <= 8192 || chunked)) ||
                                 (p->base_n == 512 && n == 1024))) {
     const size_t mul = n / 1024;
     std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);
@@ -332,6 +348,10 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
     return hip_error(cfft_launch::wd2_512_transform(fwd, d, batch, stride, p->d_tw,
                                                     p->d_tw_base1024, s),
                      "wd2_512_transform");
+  if ((p->n == 4096 || p->n == 8192) && p->base_n == 1024 && p->d_tw_base1024)  // Dif16
+    return hip_error(cfft_launch::depth_transform(fwd, log2n, d, batch, stride, p->d_tw,
+                                                  p->d_tw_base1024, s),
+                     "depth_transform");
   if (p->n == 2048 && p->d_tw_base1024)  // Dif16, base_n = 1024
     return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),
@@ -340,14 +360,31 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   if (!global.empty() && global.front().kind != cfft_prog::DEPTH_FWD &&
       global.front().kind != cfft_prog::DEPTH_INV)
     return run_global_stockham(p, global, d, batch, stride, s);
+  // chunks of a contiguous batch on the wave-local kernels
+  const bool chunks = p->d_tw_chunk && p->d_tw_base1024 && stride == p->n;
+  const uint32_t cl = p->chunk_log2_fwd;
+  auto run_chunks = [&]() -> int {
+    const uint64_t nc = batch << (log2n - cl);
+    if (cl == 11)
+      return hip_error(cfft_launch::w2048_transform(fwd, d, nc, 2048, p->d_tw_chunk,
+                                                    p->d_tw_base1024, s),
+                       "w2048_transform");
+    return hip_error(cfft_launch::depth_transform(fwd, cl, d, nc, (uint64_t)1 << cl, p->d_tw_chunk,
+                                                  p->d_tw_base1024, s),
+                     "depth_transform");
+  };
   if (fwd) {
     for (const Pass &P : p->fwd_global)
       CFFT_HIP(cfft_launch::depth_global(P, d, batch, log2n, stride, p->d_tw, s));
+    if (chunks) return run_chunks();
     if (p->fwd_lds.npass)
       CFFT_HIP(cfft_launch::fft_lds(p->fwd_lds, d, batch << (log2n - p->chunk_log2_fwd),
                                     log2n - p->chunk_log2_fwd, stride, p->d_tw, s));
   } else {
-    if (p->inv_lds.npass)
+    if (chunks) {
+      const int rc = run_chunks();
+      if (rc) return rc;
+    } else if (p->inv_lds.npass)
       CFFT_HIP(cfft_launch::fft_lds(p->inv_lds, d, batch << (log2n - p->chunk_log2_inv),
                                     log2n - p->chunk_log2_inv, stride, p->d_tw, s));
     for (const Pass &P : p->inv_global)

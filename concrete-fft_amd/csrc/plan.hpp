@@ -38,6 +38,11 @@ struct cfft_plan_s {
   // T[16 k p n/1024], the same values as a 1024-point plan's d_tw_r16; at
   // n = 512 read at column stride 2), for the wave-local kernels
   cfft_launch::dc64 *d_tw_base1024 = nullptr;
+  // Dif16 plans with base_n = 1024 and n > 8192: the twiddles of the last
+  // depth level above the base, T_c[i] = T[i n / c] for the chunk of c =
+  // 2^chunk_log2_fwd points (2048, 4096 or 8192), so that the chunks run on
+  // the wave-local kernels after the global depth passes
+  cfft_launch::dc64 *d_tw_chunk = nullptr;
   // r8 (register-resident Dif8, n = 1024) twiddle block: T[k*l] as [k][l]
   // (8 x 128), T[8 k p] as [k][p] (8 x 16), T[64 k p] as [k][p] (8 x 2)
   cfft_launch::dc64 *d_tw_r8 = nullptr;

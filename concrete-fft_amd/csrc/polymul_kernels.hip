@@ -306,6 +306,96 @@ k_w2048_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
+// ---- Dif16 plan with base_n = 1024, n = R * 1024 (R = 4, 8) ----
+// One transform on R waves, 8 / R transforms per 8-wave workgroup.  Forward:
+// the plan's one depth pass (DEPTH_FWD, radix R: butterfly j < 1024 reads the
+// points k * 1024 + j, its output k, times T[k j], goes to chunk bitrev(k))
+// runs in registers right after the loads and writes the chunks to LDS; one
+// barrier; wave c runs chunk c's 1024-point base transform in registers and
+// stores it.  Inverse: wave c loads and inverse-transforms chunk c into LDS;
+// one barrier; the inverse depth pass (conj T[k j] before the butterfly)
+// runs right before the stores.  The same operations as the uneven batch's
+// depth tiles, so the plan's bits.  In place: every point is read before the
+// barrier and written after it.  Chunk c lives in LDS at c * 1088 points (r16
+// padding) and doubles as the base transform's exchange buffer.
+template <int LOG2N, bool FWD>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_depth_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
+                  const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk) {
+  using S = R16<10>;
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr int R = 1 << (LOG2N - 10);
+  constexpr uint32_t TJ = n / 16;        // threads per transform
+  constexpr uint32_t G = 64 * kPW / TJ;  // transforms per workgroup
+  constexpr uint32_t NBD = 16 / R;       // depth butterflies per thread
+  constexpr uint32_t CH = S::lds_words;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  const uint32_t g = threadIdx.x / TJ, t = threadIdx.x % TJ;
+  const uint64_t tr = (uint64_t)blockIdx.x * G + g;
+  // a group past the batch works on the workgroup's first transform and
+  // stores nothing; every wave reaches both barriers
+  const bool active = tr < batch;
+  c64 *d = data + (active ? tr : (uint64_t)blockIdx.x * G) * stride;
+  c64 *X = smem + kTwWords + g * (R * CH);
+  const uint32_t c = t >> 6, l = t & 63;
+  c64 *xc = X + c * CH;
+  if (FWD) {
+    __syncthreads();  // twiddle block staged
+#pragma unroll 1
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+      c64 v[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) v[k] = ld_stream(&d[(uint32_t)k * 1024 + j]);
+      butterfly<R, true>(v);
+#pragma unroll
+      for (int k = 1; k < R; ++k) v[k] = cmul(tw[(uint32_t)k * j], v[k]);
+#pragma unroll
+      for (int k = 0; k < R; ++k) X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)] = v[k];
+    }
+    __syncthreads();
+    c64 v[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) v[m] = xc[r16_pad(l + 64 * m)];
+    r16l_transform<10, true>(v, v, xc, l, ltw);
+    if (active) {
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) st_stream(v[s], &d[c * 1024 + l + 64 * S::end_m(s)]);
+    }
+  } else {
+    c64 v[16], u[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) v[s] = ld_stream(&d[c * 1024 + l + 64 * S::end_m(s)]);
+    __syncthreads();  // twiddle block staged
+    r16_end_to_input<10>(v, u);
+    r16l_transform<10, false>(u, u, xc, l, ltw);
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+      c64 w[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) w[k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        c64 f = tw[(uint32_t)k * j];
+        f.im = -f.im;
+        w[k] = cmul(f, w[k]);
+      }
+      butterfly<R, false>(w);
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) st_stream(w[k], &d[(uint32_t)k * 1024 + j]);
+      }
+    }
+  }
+}
+
 // ---- Dif16 plan with n = base_n = 1024: NV transforms per wavefront ----
 // r16w_transform (twiddle reads shared by the NV transforms), 8 waves per
 // workgroup sharing the twiddle block staged in LDS.  In place: each wave
@@ -574,6 +664,39 @@ hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
   else
     hipLaunchKernelGGL(k_w2048_transform<false>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
+  return hipGetLastError();
+}
+
+hipError_t depth_transform(bool fwd, uint32_t log2n, dc64 *data, uint64_t batch, uint64_t stride,
+                           const dc64 *tw, const dc64 *tw_blk, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  if (log2n != 12 && log2n != 13) return hipErrorInvalidValue;
+  constexpr size_t lds = (kTwWords + 8 * R16<10>::lds_words) * sizeof(c64);
+  const void *k;
+  if (log2n == 12)
+    k = fwd ? reinterpret_cast<const void *>(k_depth_transform<12, true>)
+            : reinterpret_cast<const void *>(k_depth_transform<12, false>);
+  else
+    k = fwd ? reinterpret_cast<const void *>(k_depth_transform<13, true>)
+            : reinterpret_cast<const void *>(k_depth_transform<13, false>);
+  hipError_t e = cfft_launch::ensure_lds(k, lds);
+  if (e != hipSuccess) return e;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
+  const uint64_t per_wg = log2n == 12 ? 2 : 1;
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
+  if (log2n == 12) {
+    if (fwd)
+      hipLaunchKernelGGL((k_depth_transform<12, true>), grid, block, lds, s, d, batch, stride, t, tb);
+    else
+      hipLaunchKernelGGL((k_depth_transform<12, false>), grid, block, lds, s, d, batch, stride, t, tb);
+  } else {
+    if (fwd)
+      hipLaunchKernelGGL((k_depth_transform<13, true>), grid, block, lds, s, d, batch, stride, t, tb);
+    else
+      hipLaunchKernelGGL((k_depth_transform<13, false>), grid, block, lds, s, d, batch, stride, t, tb);
+  }
   return hipGetLastError();
 }
 

@@ -131,12 +131,15 @@ def test_unordered_bit_exact(cfft, oracle, n):
             assert np.array_equal(w, oracle.unordered_inv(y, algo, base_n)), (n, base_n, algo)
 
 
-@pytest.mark.parametrize("n,base_n", [(32768, 1024), (65536, 512), (131072, 1024)])
+@pytest.mark.parametrize("n,base_n", [(32768, 1024), (65536, 512), (65536, 1024),
+                                      (131072, 1024)])
 def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
-    # levels above one LDS chunk run as global depth passes
+    # levels above one LDS chunk run as global depth passes; with Dif16 and
+    # base_n = 1024 the chunks (4096, 8192 and 2048 points here) run on the
+    # wave-local kernels
     from cfft_amd.unordered import FftAlgo, Method, Plan
     rng = np.random.default_rng(n)
-    for algo in (2, 7):
+    for algo in (2, 6, 7):
         plan = Plan(n, Method.UserProvided(FftAlgo(algo), base_n))
         x = rand_c(rng, (2, n))
         y = x.copy()
@@ -150,7 +153,9 @@ def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
 
 @pytest.mark.parametrize("n,stride,batch,base", [(512, 530, 9, 512), (512, 512, 33, 512),
                                                  (1024, 1100, 7, 1024), (1024, 1030, 7, 512),
-                                                 (1024, 1024, 32, 512), (2048, 2050, 11, 1024)])
+                                                 (1024, 1024, 32, 512), (2048, 2050, 11, 1024),
+                                                 (4096, 4100, 5, 1024), (8192, 8200, 3, 1024),
+                                                 (16384, 16400, 2, 1024)])
 def test_device_path_with_stride(cfft, oracle, n, stride, batch, base):
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
