@@ -177,11 +177,15 @@ __device__ __forceinline__ void stage_block(const c64 *__restrict__ src, c64 *ds
   }
 }
 
+// glwe_tab (optional): the address of each GLWE of the batch (GLWE q at
+// glwe_tab[q], its k1 polynomials contiguous), in place of the contiguous
+// `glwe`; likewise out_tab in the column kernel.
 template <int LOG2N>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_extprod_rows_w(const uint64_t *__restrict__ glwe, c64 *__restrict__ S, uint64_t nrows,
-                 uint32_t levels, uint32_t base_log, const c64 *__restrict__ tw,
-                 const c64 *__restrict__ tw_blk, const c64 *__restrict__ twist) {
+k_extprod_rows_w(const uint64_t *__restrict__ glwe, const uint64_t *__restrict__ glwe_tab,
+                 c64 *__restrict__ S, uint64_t nrows, uint32_t k1, uint32_t levels,
+                 uint32_t base_log, const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk,
+                 const c64 *__restrict__ twist) {
   using R = R16<10>;
   constexpr uint32_t n = 1u << LOG2N;
   extern __shared__ c64 smem[];
@@ -199,7 +203,10 @@ k_extprod_rows_w(const uint64_t *__restrict__ glwe, c64 *__restrict__ S, uint64_
   const bool active = t < nrows;
   c64 *x = smem + kETw + wave * R::lds_words + half * R16<(LOG2N <= 10 ? LOG2N : 10)>::lds_words;
   const uint32_t lvl = (uint32_t)(t % levels);
-  const uint64_t *src = glwe + (active ? t / levels : 0) * 2ull * n;
+  const uint64_t q = active ? t / levels : 0;  // polynomial p k1 + poly
+  const uint64_t *src =
+      glwe_tab ? reinterpret_cast<const uint64_t *>(glwe_tab[q / k1]) + (q % k1) * 2ull * n
+               : glwe + q * 2ull * n;
   c64 *dst = S + t * n;
   auto digit_pt = [&](uint32_t j) -> c64 {
     return cmul(twist[j],
@@ -254,9 +261,9 @@ k_extprod_rows_w(const uint64_t *__restrict__ glwe, c64 *__restrict__ S, uint64_
 template <int LOG2N>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
-                 uint64_t *__restrict__ out, uint64_t ncols, uint32_t k1, uint32_t rows,
-                 const c64 *__restrict__ tw, const c64 *__restrict__ tw_blk,
-                 const c64 *__restrict__ untwist) {
+                 uint64_t *__restrict__ out, const uint64_t *__restrict__ out_tab,
+                 uint64_t ncols, uint32_t k1, uint32_t rows, const c64 *__restrict__ tw,
+                 const c64 *__restrict__ tw_blk, const c64 *__restrict__ untwist) {
   using R = R16<10>;
   constexpr uint32_t n = 1u << LOG2N;
   extern __shared__ c64 smem[];
@@ -275,7 +282,8 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   const uint64_t p = active ? t / k1 : 0;
   const uint32_t c = (uint32_t)(t % k1);
   const c64 *sp = S + p * (uint64_t)rows * n;
-  uint64_t *o = out + t * 2ull * n;
+  uint64_t *o = out_tab ? reinterpret_cast<uint64_t *>(out_tab[p]) + (uint64_t)c * 2ull * n
+                        : out + (active ? t : 0) * 2ull * n;
   auto put = [&](uint32_t j, c64 v) {
     if (!active) return;
     const c64 r = cmul(untwist[j], v);
@@ -327,10 +335,10 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
 }
 
 template <int LOG2N>
-hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, uint64_t batch,
-                          uint32_t k1, uint32_t levels, uint32_t base_log, const c64 *tw,
-                          const c64 *tw_blk, const c64 *twist, const c64 *untwist, c64 *S,
-                          hipStream_t s) {
+hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, const uint64_t *glwe_tab,
+                          uint64_t *out, const uint64_t *out_tab, uint64_t batch, uint32_t k1,
+                          uint32_t levels, uint32_t base_log, const c64 *tw, const c64 *tw_blk,
+                          const c64 *twist, const c64 *untwist, c64 *S, hipStream_t s) {
   hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_extprod_rows_w<LOG2N>),
                                          split_w_lds(8));
   if (e == hipSuccess)
@@ -356,13 +364,13 @@ hipError_t launch_split_w(const c64 *keyf, const uint64_t *glwe, uint64_t *out, 
   const uint64_t wrows = (nrows + G - 1) / G, wcols = (ncols + G - 1) / G;  // waves needed
   const uint32_t wr = waves_for(wrows, 8), wc = waves_for(wcols, 4);
   hipLaunchKernelGGL(k_extprod_rows_w<LOG2N>, dim3((uint32_t)((wrows + wr - 1) / wr)),
-                     dim3(64 * wr), split_w_lds(wr), s, glwe, S, nrows, levels, base_log, tw,
-                     tw_blk, twist);
+                     dim3(64 * wr), split_w_lds(wr), s, glwe, glwe_tab, S, nrows, k1, levels,
+                     base_log, tw, tw_blk, twist);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_extprod_cols_w<LOG2N>, dim3((uint32_t)((wcols + wc - 1) / wc)),
-                     dim3(64 * wc), split_w_lds(wc), s, keyf, S, out, ncols, k1, k1 * levels, tw,
-                     tw_blk, untwist);
+                     dim3(64 * wc), split_w_lds(wc), s, keyf, S, out, out_tab, ncols, k1,
+                     k1 * levels, tw, tw_blk, untwist);
   return hipGetLastError();
 }
 
@@ -464,23 +472,28 @@ hipError_t external_product_split(const Program &fwd, const Program &inv, const 
 
 // Split form on the wave-local kernels: Dif16 plans with n = 1024 (tw_blk =
 // the plan's r16 block), n = 512 or n = 2048 and base_n = 1024 (the plan's
-// 1024-point block).
+// 1024-point block).  glwe_tab / out_tab (optional, device-readable): one
+// address per GLWE of the batch in place of the contiguous glwe / out.
 hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint64_t *glwe,
-                                    uint64_t *out, uint64_t batch, uint32_t k1, uint32_t levels,
-                                    uint32_t base_log, const dc64 *tw, const dc64 *tw_blk,
-                                    const dc64 *twist, const dc64 *untwist, dc64 *scratch,
-                                    hipStream_t s) {
+                                    const uint64_t *glwe_tab, uint64_t *out,
+                                    const uint64_t *out_tab, uint64_t batch, uint32_t k1,
+                                    uint32_t levels, uint32_t base_log, const dc64 *tw,
+                                    const dc64 *tw_blk, const dc64 *twist, const dc64 *untwist,
+                                    dc64 *scratch, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   const c64 *kf = reinterpret_cast<const c64 *>(keyf);
   const c64 *t = reinterpret_cast<const c64 *>(tw), *tb = reinterpret_cast<const c64 *>(tw_blk);
   const c64 *z = reinterpret_cast<const c64 *>(twist), *u = reinterpret_cast<const c64 *>(untwist);
   c64 *S = reinterpret_cast<c64 *>(scratch);
   if (log2n == 9)
-    return launch_split_w<9>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
+    return launch_split_w<9>(kf, glwe, glwe_tab, out, out_tab, batch, k1, levels, base_log, t,
+                                tb, z, u, S, s);
   if (log2n == 10)
-    return launch_split_w<10>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
+    return launch_split_w<10>(kf, glwe, glwe_tab, out, out_tab, batch, k1, levels, base_log, t,
+                                tb, z, u, S, s);
   if (log2n == 11)
-    return launch_split_w<11>(kf, glwe, out, batch, k1, levels, base_log, t, tb, z, u, S, s);
+    return launch_split_w<11>(kf, glwe, glwe_tab, out, out_tab, batch, k1, levels, base_log, t,
+                                tb, z, u, S, s);
   return hipErrorInvalidValue;
 }
 

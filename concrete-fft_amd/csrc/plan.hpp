@@ -66,8 +66,16 @@ size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int level
 // holds external_product_scratch_bytes(...) bytes, else the scratch cache.
 // With no_alloc the batch runs in passes of as many GLWEs as `scratch` holds
 // and nothing is allocated (CFFT_ERROR_SIZE if one GLWE does not fit).
+// glwe_tab / out_tab (optional, device-readable, one u64 address per GLWE of
+// the batch) replace the contiguous glwe / out; they need a plan for which
+// external_product_tables_supported holds.
 int external_product_with_scratch(cfft_plan_s *plan, const void *key_fourier, int k1, int levels,
                                   int base_log, const uint64_t *glwe, uint64_t *out,
                                   size_t batch, hipStream_t stream, void *scratch,
-                                  size_t scratch_bytes, bool no_alloc);
+                                  size_t scratch_bytes, bool no_alloc,
+                                  const uint64_t *glwe_tab = nullptr,
+                                  const uint64_t *out_tab = nullptr);
+// the plan runs the wave-local split kernels (Dif16, n = 512, 1024, 2048),
+// which read and write GLWEs through address tables
+bool external_product_tables_supported(const cfft_plan_s *plan);
 }  // namespace cfft_internal

@@ -32,10 +32,11 @@ hipError_t external_product_split(const cfft_prog::Program &fwd, const cfft_prog
                                   uint32_t base_log, const dc64 *tw, const dc64 *twist,
                                   const dc64 *untwist, dc64 *scratch, hipStream_t s);
 hipError_t external_product_split_w(uint32_t log2n, const dc64 *keyf, const uint64_t *glwe,
-                                    uint64_t *out, uint64_t batch, uint32_t k1, uint32_t levels,
-                                    uint32_t base_log, const dc64 *tw, const dc64 *tw_blk,
-                                    const dc64 *twist, const dc64 *untwist, dc64 *scratch,
-                                    hipStream_t s);
+                                    const uint64_t *glwe_tab, uint64_t *out,
+                                    const uint64_t *out_tab, uint64_t batch, uint32_t k1,
+                                    uint32_t levels, uint32_t base_log, const dc64 *tw,
+                                    const dc64 *tw_blk, const dc64 *twist, const dc64 *untwist,
+                                    dc64 *scratch, hipStream_t s);
 }
 
 namespace {
@@ -79,10 +80,24 @@ size_t external_product_scratch_bytes(const cfft_plan_s *plan, int k1, int level
   return m * (uint64_t)k1 * (uint64_t)levels * plan->n * sizeof(cfft_launch::dc64);
 }
 
+// the twiddle block of the wave-local split kernels, or null when the plan
+// runs the generic split form
+const cfft_launch::dc64 *split_w_block(const cfft_plan_s *plan) {
+  return plan->n == 1024 && plan->d_tw_r16        ? plan->d_tw_r16
+         : plan->n == 512 && plan->d_tw_base1024  ? plan->d_tw_base1024
+         : plan->n == 2048 && plan->d_tw_base1024 ? plan->d_tw_base1024
+                                                  : nullptr;
+}
+
+bool external_product_tables_supported(const cfft_plan_s *plan) {
+  return plan && plan->kind == cfft_plan_s::UNORDERED && split_w_block(plan) != nullptr;
+}
+
 int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k1, int levels,
                                   int base_log, const uint64_t *glwe, uint64_t *out,
                                   size_t batch, hipStream_t stream, void *scratch,
-                                  size_t scratch_bytes, bool no_alloc) {
+                                  size_t scratch_bytes, bool no_alloc, const uint64_t *glwe_tab,
+                                  const uint64_t *out_tab) {
   int st = check_extprod_plan(plan);
   if (st != CFFT_SUCCESS) return st;
   if (k1 < 1 || k1 > kMaxK1 || levels < 1)
@@ -91,7 +106,11 @@ int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k
     return set_error(CFFT_ERROR_INVALID_ARGUMENT,
                      "gadget base_log must be in [1, 32] with levels * base_log <= 64");
   if (batch == 0) return CFFT_SUCCESS;
-  if (!key_fourier || !glwe || !out) return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  if (!key_fourier || !(glwe || glwe_tab) || !(out || out_tab))
+    return set_error(CFFT_ERROR_NULL_POINTER, "null buffer");
+  if ((glwe_tab || out_tab) && !external_product_tables_supported(plan))
+    return set_error(CFFT_ERROR_WRONG_PLAN_KIND,
+                     "address tables need the wave-local split kernels (Dif16, N = 1024 ... 4096)");
   // Rows (digit polynomials) and columns spread over workgroups through a
   // Fourier scratch: measured faster than one workgroup per GLWE from batch 1
   // to 4096 at N = 1024, 2048 and 4096.
@@ -120,22 +139,20 @@ int external_product_with_scratch(cfft_plan plan, const void *key_fourier, int k
   const size_t glwe_words = (size_t)k1 * 2 * plan->n;  // one GLWE: k1 polynomials of N words
   for (uint64_t off = 0; off < batch && e == hipSuccess; off += per_pass) {
     const uint64_t m = batch - off < per_pass ? batch - off : per_pass;
-    const uint64_t *g = glwe + off * glwe_words;
-    uint64_t *o = out + off * glwe_words;
+    const uint64_t *g = glwe ? glwe + off * glwe_words : nullptr;
+    uint64_t *o = out ? out + off * glwe_words : nullptr;
+    const uint64_t *gt = glwe_tab ? glwe_tab + off : nullptr;
+    const uint64_t *ot = out_tab ? out_tab + off : nullptr;
     // Dif16 plans with n = 512, 1024 and 2048 (base 1024): one wave per row
     // and per column (wave-local register-resident transforms; two per wave
     // at n = 512).  Measured at N = 4096 against two waves per row transform
     // in the generic split form: 45 vs 72 us for 16 GLWEs, 155 vs 402 us for
     // 1024, 0.86 vs 1.52 ms for 4096.
-    const cfft_launch::dc64 *blk =
-        plan->n == 1024 && plan->d_tw_r16         ? plan->d_tw_r16
-        : plan->n == 512 && plan->d_tw_base1024   ? plan->d_tw_base1024
-        : plan->n == 2048 && plan->d_tw_base1024  ? plan->d_tw_base1024
-                                                   : nullptr;
+    const cfft_launch::dc64 *blk = split_w_block(plan);
     if (blk)
       e = cfft_launch::external_product_split_w(
-          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), g, o, m,
-          (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk, plan->d_twist,
+          ilog2(plan->n), reinterpret_cast<const cfft_launch::dc64 *>(key_fourier), g, gt, o, ot,
+          m, (uint32_t)k1, (uint32_t)levels, (uint32_t)base_log, plan->d_tw, blk, plan->d_twist,
           plan->d_untwist, S, s);
     else
       e = cfft_launch::external_product_split(

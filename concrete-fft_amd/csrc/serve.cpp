@@ -10,10 +10,12 @@
 //      requests -- the group (kind, N, key) of the oldest request, in arrival
 //      order, as many as fit the batch limit and the free device / staging
 //      memory -- and launches it: one host-to-device copy, one batched kernel,
-//      one device-to-host copy, one event (device-buffer requests: a gather
-//      kernel reads the inputs from the callers' device buffers, named by a
-//      table of addresses in mapped pinned staging, and a scatter kernel
-//      writes the results back; nothing crosses the bus but the table);
+//      one device-to-host copy, one event (device-buffer requests: the
+//      callers' device buffers are named by a table of addresses in mapped
+//      pinned staging; the wave-local external-product kernels read and
+//      write them through it, other batches are gathered into the batch's
+//      block by a kernel and scattered back by another; nothing crosses the
+//      bus but the table);
 //   3. sleeps until a submission arrives, or polls while batches are in flight.
 // So requests that arrive while others run on the device are admitted as soon
 // as a slot and memory free up, and batches on different streams overlap.
@@ -258,18 +260,24 @@ hipError_t cfft_server_s::launch(Batch &b) {
   const size_t N = (size_t)1 << b.log2N;
   const size_t in_rows = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? m : 2 * m;
   const size_t in_row_bytes = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? b.reqs[0].in_bytes : N * 8;
-  hipError_t e;
+  hipError_t e = hipSuccess;
+  // device-buffer external products on the wave-local split kernels read
+  // and write the callers' buffers through the address table directly
+  const bool direct = b.dev && b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT &&
+                      cfft_internal::external_product_tables_supported(keys[b.key].plan);
+  const uint64_t *tab_dev = reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off);
   if (b.dev) {
     // address table: input rows (products: all A rows, then all B rows),
-    // then output rows; the gather reads it over the bus
+    // then output rows; the kernels read it over the bus
     uint64_t *tab = reinterpret_cast<uint64_t *>(h_in);
     for (size_t i = 0; i < m; ++i) {
       tab[i] = (uint64_t)(uintptr_t)b.reqs[i].in0;
       if (b.kind == CFFT_REQUEST_POLYMUL) tab[m + i] = (uint64_t)(uintptr_t)b.reqs[i].in1;
       tab[in_rows + i] = (uint64_t)(uintptr_t)b.reqs[i].out;
     }
-    e = cfft_launch::gather_rows(reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off),
-                                 in_rows, in_row_bytes / 8, reinterpret_cast<uint64_t *>(d_in), st);
+    if (!direct)
+      e = cfft_launch::gather_rows(tab_dev, in_rows, in_row_bytes / 8,
+                                   reinterpret_cast<uint64_t *>(d_in), st);
   } else {
     // gather inputs into pinned staging
     for (size_t i = 0; i < m; ++i) {
@@ -288,8 +296,10 @@ hipError_t cfft_server_s::launch(Batch &b) {
     // its outputs: no allocation and no shared buffer on the serving path
     const size_t io = round_up(b.in_bytes) + round_up(m * b.reqs[0].out_bytes);
     rc = cfft_internal::external_product_with_scratch(
-        k.plan, k.d_fourier, k.k1, k.levels, k.base_log, reinterpret_cast<const uint64_t *>(d_in),
-        reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io, false);
+        k.plan, k.d_fourier, k.k1, k.levels, k.base_log,
+        direct ? nullptr : reinterpret_cast<const uint64_t *>(d_in),
+        direct ? nullptr : reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io,
+        false, direct ? tab_dev : nullptr, direct ? tab_dev + in_rows : nullptr);
   } else {
     cfft_plan p = plans[b.log2N].load(std::memory_order_acquire);
     rc = cfft_negacyclic_polymul_u64(p, reinterpret_cast<const uint64_t *>(d_in),
@@ -299,9 +309,9 @@ hipError_t cfft_server_s::launch(Batch &b) {
   if (rc != CFFT_SUCCESS) return hipErrorLaunchFailure;
   const size_t per_out = b.reqs[0].out_bytes;
   if (b.dev) {
-    e = cfft_launch::scatter_rows(
-        reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off) + in_rows, m, per_out / 8,
-        reinterpret_cast<const uint64_t *>(d_out), st);
+    if (!direct)
+      e = cfft_launch::scatter_rows(tab_dev + in_rows, m, per_out / 8,
+                                    reinterpret_cast<const uint64_t *>(d_out), st);
   } else {
     e = hipMemcpyAsync(h_pool + b.host_off + round_up(b.in_bytes), d_out, m * per_out,
                        hipMemcpyDeviceToHost, st);

@@ -270,11 +270,14 @@ def _make_device(server, keys, r):
 def test_device_requests_alone_equals_mix_equals_host(cfft):
     """Device-buffer requests: alone, in a burst mixed with host-buffer
     requests of the same shapes, and against the host-buffer results (which
-    equal the direct entry points, test above), bit for bit."""
+    equal the direct entry points, test above), bit for bit.  External
+    products at N = 1024 ... 4096 read and write the callers' buffers through
+    the address table; N = 512 and 8192 and the products are gathered and
+    scattered."""
     from cfft_amd.serve import Server
     rng = np.random.default_rng(15)
     srv = Server()
-    keys = _setup(srv, rng)
+    keys = _setup(srv, rng, sizes=(512, 1024, 2048, 4096, 8192))
     reqs = _requests(srv, keys, rng, 48)
     host = []
     for r in reqs:
