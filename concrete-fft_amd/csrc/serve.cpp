@@ -56,6 +56,11 @@ constexpr int kMaxKeys = 256;
 constexpr int kMaxLog2N = 14;
 // rows of one gather / scatter launch (two input rows per product request)
 constexpr size_t kMaxDeviceBatch = 32767;
+// host-buffer external-product batches up to this size are read and written
+// by the wave-local kernels in the mapped pinned staging itself (no DMA
+// copies: each input is read once per gadget level over the bus, which
+// larger batches pay more for than two copies)
+constexpr size_t kHostDirectMax = 16;
 
 size_t round_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
@@ -261,11 +266,14 @@ hipError_t cfft_server_s::launch(Batch &b) {
   const size_t in_rows = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? m : 2 * m;
   const size_t in_row_bytes = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? b.reqs[0].in_bytes : N * 8;
   hipError_t e = hipSuccess;
-  // device-buffer external products on the wave-local split kernels read
-  // and write the callers' buffers through the address table directly
-  const bool direct = b.dev && b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT &&
+  // external products on the wave-local split kernels read and write
+  // through an address table: the callers' device buffers, or for a small
+  // host-buffer batch the request slots of the mapped pinned staging
+  const bool tables = b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT &&
                       cfft_internal::external_product_tables_supported(keys[b.key].plan);
-  const uint64_t *tab_dev = reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off);
+  const bool direct = tables && (b.dev || m <= kHostDirectMax);
+  const size_t host_tab_off = b.dev ? 0 : round_up(b.in_bytes) + round_up(m * b.reqs[0].out_bytes);
+  const uint64_t *tab_dev = reinterpret_cast<const uint64_t *>(h_pool_dev + b.host_off + host_tab_off);
   if (b.dev) {
     // address table: input rows (products: all A rows, then all B rows),
     // then output rows; the kernels read it over the bus
@@ -285,7 +293,16 @@ hipError_t cfft_server_s::launch(Batch &b) {
       if (b.kind == CFFT_REQUEST_POLYMUL)
         memcpy(h_in + (m + i) * in_row_bytes, b.reqs[i].in1, in_row_bytes);
     }
-    e = hipMemcpyAsync(d_in, h_in, b.in_bytes, hipMemcpyHostToDevice, st);
+    if (direct) {
+      uint64_t *tab = reinterpret_cast<uint64_t *>(h_in + host_tab_off);
+      const uint64_t base = (uint64_t)(uintptr_t)(h_pool_dev + b.host_off);
+      for (size_t i = 0; i < m; ++i) {
+        tab[i] = base + i * in_row_bytes;
+        tab[m + i] = base + round_up(b.in_bytes) + i * b.reqs[0].out_bytes;
+      }
+    } else {
+      e = hipMemcpyAsync(d_in, h_in, b.in_bytes, hipMemcpyHostToDevice, st);
+    }
   }
   if (e != hipSuccess) return e;
   char *d_out = d_in + round_up(b.in_bytes);
@@ -299,7 +316,7 @@ hipError_t cfft_server_s::launch(Batch &b) {
         k.plan, k.d_fourier, k.k1, k.levels, k.base_log,
         direct ? nullptr : reinterpret_cast<const uint64_t *>(d_in),
         direct ? nullptr : reinterpret_cast<uint64_t *>(d_out), m, st, d_in + io, b.dev_len - io,
-        false, direct ? tab_dev : nullptr, direct ? tab_dev + in_rows : nullptr);
+        false, direct ? tab_dev : nullptr, direct ? tab_dev + m : nullptr);
   } else {
     cfft_plan p = plans[b.log2N].load(std::memory_order_acquire);
     rc = cfft_negacyclic_polymul_u64(p, reinterpret_cast<const uint64_t *>(d_in),
@@ -312,7 +329,7 @@ hipError_t cfft_server_s::launch(Batch &b) {
     if (!direct)
       e = cfft_launch::scatter_rows(tab_dev + in_rows, m, per_out / 8,
                                     reinterpret_cast<const uint64_t *>(d_out), st);
-  } else {
+  } else if (!direct) {
     e = hipMemcpyAsync(h_pool + b.host_off + round_up(b.in_bytes), d_out, m * per_out,
                        hipMemcpyDeviceToHost, st);
   }
@@ -366,9 +383,12 @@ bool cfft_server_s::launch_one() {
   b.slot = slot;
   const size_t limit = b.dev ? std::min<size_t>(cfg.max_batch, kMaxDeviceBatch) : cfg.max_batch;
   // pinned staging of a batch: its data (host requests, inputs then
-  // outputs) or its address table (device requests)
-  auto host_need = [&](size_t in_sum, size_t out_sum, size_t tab_sum) {
-    return b.dev ? round_up(tab_sum) : round_up(in_sum) + round_up(out_sum);
+  // outputs, then for external products an address table of 2 words per
+  // request) or its address table (device requests)
+  const bool host_tab = !b.dev && b.kind == CFFT_REQUEST_EXTERNAL_PRODUCT;
+  auto host_need = [&](size_t in_sum, size_t out_sum, size_t tab_sum, size_t count) {
+    if (b.dev) return round_up(tab_sum);
+    return round_up(in_sum) + round_up(out_sum) + (host_tab ? round_up(16 * count) : 0);
   };
   // collect the head's group in arrival order while memory allows
   size_t in_sum = 0, out_sum = 0, tab_sum = 0;
@@ -379,7 +399,7 @@ bool cfft_server_s::launch_one() {
     const size_t need = round_up(in_sum + r.in_bytes) + round_up(out_sum + r.out_bytes);
     const size_t dneed = need + dev_scratch(b, take.size() + 1);
     const size_t hneed = host_need(in_sum + r.in_bytes, out_sum + r.out_bytes,
-                                   tab_sum + r.host_bytes);
+                                   tab_sum + r.host_bytes, take.size() + 1);
     if (dneed > dalloc.capacity() || hneed > halloc.capacity()) break;
     take.push_back(i);
     in_sum += r.in_bytes;
@@ -389,7 +409,7 @@ bool cfft_server_s::launch_one() {
   // shrink until both pools have room (admission by free memory)
   while (!take.empty()) {
     const size_t dneed = round_up(in_sum) + round_up(out_sum) + dev_scratch(b, take.size());
-    const size_t hneed = host_need(in_sum, out_sum, tab_sum);
+    const size_t hneed = host_need(in_sum, out_sum, tab_sum, take.size());
     size_t doff = 0, hoff = 0;
     if (dalloc.alloc(dneed, &doff)) {
       if (halloc.alloc(hneed, &hoff)) {
@@ -637,7 +657,8 @@ int cfft_server_submit(cfft_server s, const cfft_request *r, uint64_t *ticket) {
                        : q.in_bytes + q.out_bytes;
   const size_t need = round_up(q.in_bytes) + round_up(q.out_bytes);
   const size_t dneed = need + (k ? key_scratch(*k, 1) : 0);
-  const size_t hneed = q.dev ? round_up(q.host_bytes) : need;
+  const size_t hneed = q.dev ? round_up(q.host_bytes)
+                            : need + (q.kind == CFFT_REQUEST_EXTERNAL_PRODUCT ? round_up(16) : 0);
   if (dneed > s->cfg.hbm_budget_bytes || hneed > s->cfg.pinned_bytes)
     return set_error(CFFT_ERROR_BUSY, "request larger than the server's memory budget");
   {
