@@ -344,3 +344,27 @@ def test_device_request_refuses_host_memory(cfft):
         srv.submit(r, (a, b, out))
     assert ei.value.status == 10  # CFFT_ERROR_INVALID_ARGUMENT
     srv.close()
+
+
+def test_large_host_batches(cfft):
+    """One launch slot and 96 external products submitted at once (open-loop
+    replay, all arrivals at 0): the requests queued behind the first batch
+    form batches above 16, which copy through the device; every result
+    equals the same request served alone."""
+    from cfft_amd.serve import Server
+    rng = np.random.default_rng(17)
+    srv = Server(streams=1)
+    keys = _setup(srv, rng, sizes=(2048,))
+    reqs = [("extprod", 2048, rng.integers(0, M, (K1, 2048), dtype=np.uint64)) for _ in range(96)]
+    made = [_make(srv, keys, r) for r in reqs]
+    before = srv.stats()
+    srv.replay([q for q, _ in made], np.zeros(len(made)))
+    after = srv.stats()
+    batches = after["batches"] - before["batches"]
+    print("96 requests in %d batches" % batches)
+    assert batches < 96 // 4
+    for i, r in enumerate(reqs):
+        q, keep = _make(srv, keys, r)
+        alone = srv.wait(srv.submit(q, keep))
+        assert np.array_equal(made[i][1][-1], alone), i
+    srv.close()
