@@ -134,6 +134,47 @@ def cpu_baseline(a, b, algo, base_n, target_s=4.0):
             "series": series, "series_identical": same}, results[best], BATCH
 
 
+def cpu_transforms(algo, base_n, n=1024, target_s=2.0):
+    """Unordered forward transforms per second on the host: the AVX-512 (V4)
+    and AVX2 (V3) restatements of the reference plan (n = 1024, the headline
+    product's transform), one Python thread per host core each calling the
+    batched C routine on its own 512 transforms (ctypes releases the GIL
+    during the call), for about target_s seconds per series."""
+    import threading
+    from oracle_lib import load_oracle
+    O = load_oracle()
+    threads = host_cores()
+    out = {}
+    for name, isa in (("avx512", 4), ("avx2", 3)):
+        if not O.L.or_cpu_has_isa(isa):
+            out[name] = {"value": None, "note": "host CPU lacks this instruction set"}
+            continue
+        f = getattr(O.L, "or_simd_unordered_fwd_batch_v%d" % isa)
+        per = 512
+        bufs = [np.random.default_rng(k).uniform(-1, 1, 2 * n * per) for k in range(threads)]
+        counts = [0] * threads
+        t_end = [0.0]
+
+        def work(k):
+            while time.perf_counter() < t_end[0]:
+                if f(n, int(algo), base_n, bufs[k].ctypes.data, per) != 0:
+                    return
+                counts[k] += per
+        t0 = time.perf_counter()
+        t_end[0] = t0 + target_s
+        ths = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        wall = time.perf_counter() - t0
+        out[name] = {"value": round(sum(counts) / wall, 1), "transforms": sum(counts),
+                     "seconds": round(wall, 2)}
+    return {"unit": "transforms/s", "n": n, "cores": threads, "kind": "port",
+            "form": "unordered fwd, plan %d / base %d, one thread per core" % (algo, base_n),
+            "series": out}
+
+
 UNEVEN_JOBS = 20000
 UNEVEN_LOG2N = [9, 10, 11, 12, 13, 14]
 UNEVEN_WEIGHTS = [0.10, 0.20, 0.35, 0.20, 0.10, 0.05]
@@ -703,6 +744,7 @@ def main():
 
     if not args.no_cpu and args.mode == "all":
         cpu, c_cpu, ns = cpu_baseline(a, b, args.algo, args.base_n)
+        cpu["transforms"] = cpu_transforms(args.algo, args.base_n)
         out["cpu_baseline"] = cpu
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
         out["gpu_equals_cpu_bitwise"] = bool(np.array_equal(c_cpu, c_gpu[:ns]))
