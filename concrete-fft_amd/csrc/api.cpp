@@ -935,6 +935,11 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && p->n == 1024 && p->d_tw_r16)
     return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw_r16, p->d_twist, s),
                      "r16_polymul");
+  if (p->algo == CFFT_DIF16 && p->base_n == 1024 && p->n >= 2048 && p->n <= 8192 &&
+      p->d_tw_base1024)
+    return hip_error(cfft_launch::depth_polymul(log2n, a, b, c, batch, p->d_tw, p->d_tw_base1024,
+                                                p->d_twist, p->d_untwist, s),
+                     "depth_polymul");
   if (p->d_tw_r8)
     return hip_error(cfft_launch::r8_polymul(a, b, c, batch, p->d_tw_r8, p->d_twist, s),
                      "r8_polymul");

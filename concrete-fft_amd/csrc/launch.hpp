@@ -86,6 +86,12 @@ hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
 // 1024-point r16 block).
 hipError_t depth_transform(bool fwd, uint32_t log2n, dc64 *data, uint64_t batch, uint64_t stride,
                            const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
+// Fused negacyclic product on Dif16 plans with base_n = 1024 and n = 2048,
+// 4096 or 8192 (log2n 11 .. 13): the depth-pass structure above for both
+// operands and the inverse (twist / untwist tables of the plan).
+hipError_t depth_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
+                         uint64_t batch, const dc64 *tw, const dc64 *tw_blk, const dc64 *twist,
+                         const dc64 *untwist, hipStream_t s);
 hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
                            const dc64 *tw_blk, hipStream_t s);
 

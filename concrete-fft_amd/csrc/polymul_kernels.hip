@@ -396,6 +396,118 @@ k_depth_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
+// ---- negacyclic product, Dif16 plan with base_n = 1024, n = R * 1024 ----
+// (R = 2, 4, 8: N = 4096, 8192, 16384.)  One product on R waves, 8 / R
+// products per 8-wave workgroup, each operand through the structure of
+// k_depth_transform: fold + twist + the plan's depth pass in registers after
+// the loads, chunks to LDS, one 1024-point base transform per wave.  A's
+// spectrum stays in registers while B's is formed; the pointwise product,
+// the inverse base transform, the inverse depth pass, untwist (with 1/n)
+// and torus rounding follow before the stores.  The same operations in the
+// same order as the generic fused product (k_polymul_generic), so the same
+// bits.
+template <int LOG2N>
+__device__ __forceinline__ void depth_pass_to_lds(c64 *X, uint32_t t, const c64 *tw,
+                                                  const c64 *twist, const uint64_t *a,
+                                                  const int64_t *b) {
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr int R = 1 << (LOG2N - 10);
+  constexpr uint32_t TJ = n / 16, NBD = 16 / R, CH = R16<10>::lds_words;
+#pragma unroll 1
+  for (uint32_t i = 0; i < NBD; ++i) {
+    const uint32_t j = t + TJ * i;
+    c64 v[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint32_t q = (uint32_t)k * 1024 + j;
+      const c64 x = a ? mk(torus_to_f64(ld_stream(&a[q])), torus_to_f64(ld_stream(&a[q + n])))
+                      : mk(i64_to_f64(ld_stream(&b[q])), i64_to_f64(ld_stream(&b[q + n])));
+      v[k] = cmul(twist[q], x);
+    }
+    butterfly<R, true>(v);
+#pragma unroll
+    for (int k = 1; k < R; ++k) v[k] = cmul(tw[(uint32_t)k * j], v[k]);
+#pragma unroll
+    for (int k = 0; k < R; ++k) X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)] = v[k];
+  }
+}
+
+template <int LOG2N>
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_depth_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+                uint64_t *__restrict__ C, uint64_t batch, const c64 *__restrict__ tw,
+                const c64 *__restrict__ tw_blk, const c64 *__restrict__ twist,
+                const c64 *__restrict__ untwist) {
+  using S = R16<10>;
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr int R = 1 << (LOG2N - 10);
+  constexpr uint32_t TJ = n / 16;        // threads per product
+  constexpr uint32_t G = 64 * kPW / TJ;  // products per workgroup
+  constexpr uint32_t NBD = 16 / R;
+  constexpr uint32_t CH = S::lds_words;
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  const uint32_t g = threadIdx.x / TJ, t = threadIdx.x % TJ;
+  const uint64_t pr = (uint64_t)blockIdx.x * G + g;
+  // a group past the batch works on the workgroup's first product and stores
+  // nothing; every wave reaches every barrier
+  const bool active = pr < batch;
+  const uint64_t off = (active ? pr : (uint64_t)blockIdx.x * G) * (2ull * n);
+  c64 *X = smem + kTwWords + g * (R * CH);
+  const uint32_t c = t >> 6, l = t & 63;
+  c64 *xc = X + c * CH;
+  __syncthreads();  // twiddle block staged
+  depth_pass_to_lds<LOG2N>(X, t, tw, twist, A + off, nullptr);
+  __syncthreads();
+  c64 fa[16];
+#pragma unroll
+  for (uint32_t m = 0; m < 16; ++m) fa[m] = xc[r16_pad(l + 64 * m)];
+  r16l_transform<10, true>(fa, fa, xc, l, ltw);
+  __syncthreads();  // every chunk read before B's depth pass overwrites it
+  depth_pass_to_lds<LOG2N>(X, t, tw, twist, nullptr, B + off);
+  __syncthreads();
+  {
+    c64 fb[16];
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) fb[m] = xc[r16_pad(l + 64 * m)];
+    r16l_transform<10, true>(fb, fb, xc, l, ltw);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], fb[s]);
+  }
+  c64 u[16];
+  r16_end_to_input<10>(fa, u);
+  r16l_transform<10, false>(u, u, xc, l, ltw);
+  wave_lds_fence();
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
+  __syncthreads();
+  uint64_t *cp = C + off;
+#pragma unroll 1
+  for (uint32_t i = 0; i < NBD; ++i) {
+    const uint32_t j = t + TJ * i;
+    c64 w[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) w[k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
+#pragma unroll
+    for (int k = 1; k < R; ++k) {
+      c64 f = tw[(uint32_t)k * j];
+      f.im = -f.im;
+      w[k] = cmul(f, w[k]);
+    }
+    butterfly<R, false>(w);
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t q = (uint32_t)k * 1024 + j;
+        const c64 r = cmul(untwist[q], w[k]);
+        st_stream(from_torus(r.re), &cp[q]);
+        st_stream(from_torus(r.im), &cp[q + n]);
+      }
+    }
+  }
+}
+
 // ---- Dif16 plan with n = base_n = 1024: NV transforms per wavefront ----
 // r16w_transform (twiddle reads shared by the NV transforms), 8 waves per
 // workgroup sharing the twiddle block staged in LDS.  In place: each wave
@@ -697,6 +809,30 @@ hipError_t depth_transform(bool fwd, uint32_t log2n, dc64 *data, uint64_t batch,
     else
       hipLaunchKernelGGL((k_depth_transform<13, false>), grid, block, lds, s, d, batch, stride, t, tb);
   }
+  return hipGetLastError();
+}
+
+hipError_t depth_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
+                         uint64_t batch, const dc64 *tw, const dc64 *tw_blk, const dc64 *twist,
+                         const dc64 *untwist, hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  if (log2n < 11 || log2n > 13) return hipErrorInvalidValue;
+  constexpr size_t lds = (kTwWords + 8 * R16<10>::lds_words) * sizeof(c64);
+  const void *k = log2n == 11   ? reinterpret_cast<const void *>(k_depth_polymul<11>)
+                  : log2n == 12 ? reinterpret_cast<const void *>(k_depth_polymul<12>)
+                                : reinterpret_cast<const void *>(k_depth_polymul<13>);
+  hipError_t e = cfft_launch::ensure_lds(k, lds);
+  if (e != hipSuccess) return e;
+  const c64 *t = reinterpret_cast<const c64 *>(tw), *tb = reinterpret_cast<const c64 *>(tw_blk);
+  const c64 *z = reinterpret_cast<const c64 *>(twist), *u = reinterpret_cast<const c64 *>(untwist);
+  const uint64_t per_wg = 8u >> (log2n - 10);  // 8 / R products
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
+  if (log2n == 11)
+    hipLaunchKernelGGL(k_depth_polymul<11>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
+  else if (log2n == 12)
+    hipLaunchKernelGGL(k_depth_polymul<12>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
+  else
+    hipLaunchKernelGGL(k_depth_polymul<13>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
   return hipGetLastError();
 }
 
