@@ -211,13 +211,15 @@ int build_f64_plan(cfft_plan_s *p) {
     CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
     CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
   }
-  if (p->algo == CFFT_DIF16 && p->base_n == 512 && n == 512) {
-    // the wave-local n = 512 transform reads the 1024-point block at column
-    // stride 2: T_1024[2 j] = T_512[j] (same sincospi argument), checked here
+  if (p->algo == CFFT_DIF16 && p->base_n == n && (n == 256 || n == 512)) {
+    // the wave-local n = 256 / 512 kernels read the 1024-point block at
+    // column stride 1024 / n: T_1024[(1024 / n) j] = T_n[j] (same sincospi
+    // argument), checked here
     const std::vector<cfft_host::C64> t1024 = cfft_host::twiddle_table(1024);
+    const size_t cs = 1024 / n;
     bool same = true;
     for (size_t j = 0; j < n && same; ++j)
-      same = std::memcmp(&t1024[2 * j], &tw[j], sizeof(cfft_host::C64)) == 0;
+      same = std::memcmp(&t1024[cs * j], &tw[j], sizeof(cfft_host::C64)) == 0;
     if (same) {
       std::vector<cfft_host::C64> blk(16 * 64 + 16 * 4);
       for (size_t k = 0; k < 16; ++k) {
@@ -929,9 +931,10 @@ int cfft_negacyclic_polymul_u64(cfft_plan plan, const uint64_t *a, const int64_t
 int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t *b, uint64_t *c,
                           size_t batch, hipStream_t s) {
   const uint32_t log2n = ilog2(p->n);
-  if (p->n == 512 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, wave-local
-    return hip_error(cfft_launch::w512_polymul(a, b, c, batch, p->d_tw_base1024, p->d_twist, s),
-                     "w512_polymul");
+  if ((p->n == 256 || p->n == 512) && p->base_n == p->n && p->d_tw_base1024)  // Dif16
+    return hip_error(cfft_launch::wsub_polymul(log2n, a, b, c, batch, p->d_tw_base1024,
+                                               p->d_twist, s),
+                     "wsub_polymul");
   if (p->algo == CFFT_DIF16 && p->base_n == p->n && p->n == 1024 && p->d_tw_r16)
     return hip_error(cfft_launch::r16_polymul(log2n, a, b, c, batch, p->d_tw_r16, p->d_twist, s),
                      "r16_polymul");

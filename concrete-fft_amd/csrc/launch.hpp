@@ -61,10 +61,11 @@ hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint
 hipError_t w512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                           const dc64 *tw_blk, hipStream_t s);
 
-// Negacyclic product (N = 1024) on a Dif16 plan with n = base_n = 512: one
-// product per 32-lane half-wave.
-hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
-                        const dc64 *tw_blk, const dc64 *twist, hipStream_t s);
+// Negacyclic product (N = 512 / 1024) on a Dif16 plan with n = base_n = 256
+// or 512 (log2n 8, 9): one product per group of n / 16 lanes; tw_blk is the
+// plan's 1024-point r16 block.
+hipError_t wsub_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
+                        uint64_t batch, const dc64 *tw_blk, const dc64 *twist, hipStream_t s);
 
 // Dif16 plans with n = 1024 and base_n = 512: one transform per 32-lane
 // half-wave (r16l_depth2_transform<9>); tw is the plan's table, tw_blk its

@@ -584,34 +584,38 @@ k_w512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
-// ---- negacyclic product, N = 1024 on a Dif16 plan with n = base_n = 512 ----
-// One product per 32-lane half-wave, the same operations as the n = 1024
-// product kernel: fold + twist at the loads, both forward transforms
-// together (r16l_transform<9>, twiddle reads shared), pointwise product,
-// inverse, untwist and torus rounding at the stores.  8 waves per workgroup
-// share the 1024-point twiddle block in LDS.  A half past the batch repeats
-// the last product and stores nothing.
+// ---- negacyclic product, N = 512 / 1024 on a Dif16 plan with n = base_n ----
+// (n = 2^LOG2N = 256 or 512.)  One product per group of T = n / 16 lanes
+// (a quarter- or half-wave), the same operations as the n = 1024 product
+// kernel: fold + twist at the loads, both forward transforms together
+// (r16l_transform<LOG2N>, twiddle reads shared), pointwise product, inverse,
+// untwist and torus rounding at the stores.  8 waves per workgroup share the
+// 1024-point twiddle block in LDS (read at column stride 1024 / n).  A group
+// past the batch repeats the last product and stores nothing.
+template <int LOG2N>
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_w512_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+k_wsub_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
                uint64_t *__restrict__ C, uint64_t batch, const c64 *__restrict__ tw_blk,
                const c64 *__restrict__ twist) {
-  using S = R16<9>;
+  using S = R16<LOG2N>;
   constexpr uint32_t n = S::n;
+  constexpr uint32_t PW = 64 / S::T;  // products per wave
+  static_assert(PW * S::lds_words <= R16<10>::lds_words, "per-wave exchange space");
   extern __shared__ c64 smem[];
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * 2;
+  const uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * PW;
   if (first >= batch) return;
-  const uint32_t half = (threadIdx.x >> 5) & 1u, l = threadIdx.x & 31u;
-  const uint64_t p = first + half;
+  const uint32_t grp = (threadIdx.x & 63u) / S::T, l = threadIdx.x % S::T;
+  const uint64_t p = first + grp;
   const bool act = p < batch;
   const uint64_t pc = act ? p : batch - 1;
   const uint64_t *a = A + pc * (2 * n);
   const int64_t *b = B + pc * (2 * n);
   uint64_t *c = C + pc * (2 * n);
-  c64 *x = smem + kTwWords + (wave * 2 + half) * S::lds_words;
+  c64 *x = smem + kTwWords + wave * R16<10>::lds_words + grp * S::lds_words;
   const double inv_n = 1.0 / (double)n;
   c64 fa[16], v[16];
 #pragma unroll
@@ -621,12 +625,12 @@ k_w512_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
     fa[m] = cmul(z, mk(torus_to_f64(ld_stream(&a[j])), torus_to_f64(ld_stream(&a[j + n]))));
     v[m] = cmul(z, mk(i64_to_f64(ld_stream(&b[j])), i64_to_f64(ld_stream(&b[j + n]))));
   }
-  r16l_transform<9, true, 2>(fa, v, x, l, ltw);
+  r16l_transform<LOG2N, true, 2>(fa, v, x, l, ltw);
   c64 z[16];
 #pragma unroll
   for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
-  r16_end_to_input<9>(fa, z);
-  r16l_transform<9, false, 1>(z, z, x, l, ltw);
+  r16_end_to_input<LOG2N>(fa, z);
+  r16l_transform<LOG2N, false, 1>(z, z, x, l, ltw);
   const c64 *tws = launder(twist);
 #pragma unroll
   for (uint32_t s = 0; s < 16; ++s) {
@@ -704,16 +708,21 @@ hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stri
   return hipGetLastError();
 }
 
-hipError_t w512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
-                        const dc64 *tw_blk, const dc64 *twist, hipStream_t s) {
+hipError_t wsub_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint64_t *c,
+                        uint64_t batch, const dc64 *tw_blk, const dc64 *twist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_w512_polymul),
-                                         kWaveLds);
+  if (log2n != 8 && log2n != 9) return hipErrorInvalidValue;
+  const void *k = log2n == 8 ? reinterpret_cast<const void *>(k_wsub_polymul<8>)
+                             : reinterpret_cast<const void *>(k_wsub_polymul<9>);
+  hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
-  const uint64_t per_wg = (uint64_t)kPW * 2;
-  hipLaunchKernelGGL(k_w512_polymul, dim3((uint32_t)((batch + per_wg - 1) / per_wg)),
-                     dim3(64 * kPW), kWaveLds, s, a, b, c, batch,
-                     reinterpret_cast<const c64 *>(tw_blk), reinterpret_cast<const c64 *>(twist));
+  const uint64_t per_wg = (uint64_t)kPW * (log2n == 8 ? 4 : 2);
+  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
+  const c64 *tb = reinterpret_cast<const c64 *>(tw_blk), *z = reinterpret_cast<const c64 *>(twist);
+  if (log2n == 8)
+    hipLaunchKernelGGL(k_wsub_polymul<8>, grid, block, kWaveLds, s, a, b, c, batch, tb, z);
+  else
+    hipLaunchKernelGGL(k_wsub_polymul<9>, grid, block, kWaveLds, s, a, b, c, batch, tb, z);
   return hipGetLastError();
 }
 
@@ -877,7 +886,7 @@ hipError_t r16_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uint
                        uint64_t batch, const dc64 *tw, const dc64 *twist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
   // n = 1024: one product per wave, 8 waves per workgroup sharing the
-  // twiddles staged in LDS (n = 512 runs k_w512_polymul)
+  // twiddles staged in LDS (n = 512 runs k_wsub_polymul)
   if (log2n != 10) return hipErrorInvalidValue;
   return launch_polymul_persistent(a, b, c, batch, tw, twist, s);
 }
