@@ -172,9 +172,12 @@ void tune_store(const std::string &device_key, int kind, size_t n, const TuneCho
   g_table[Key{device_key, kind, n}] = c;
   const std::string path = default_path();
   if (path.empty()) return;
-  // merge what other processes wrote since this one read the file
-  read_file(path, g_table, true, nullptr);
-  (void)write_file(path, g_table);
+  // the file keeps what other processes wrote since this one read it; the
+  // process table is not refilled from it (a cleared table stays cleared)
+  Table merged;
+  read_file(path, merged, false, nullptr);
+  for (const auto &kv : g_table) merged[kv.first] = kv.second;
+  (void)write_file(path, merged);
 }
 
 }  // namespace cfft_internal

@@ -130,3 +130,16 @@ def test_long_lines_are_dropped(tmp_path):
     _write(p, ["x" * 10000 + "\t1\t1024\t6\t1024", "%s\t1\t1024\t6\t1024" % KEY])
     assert tuning.import_file(p) == 1
     assert tuning.lookup(KEY, tuning.UNORDERED, 1024) == (6, 1024)
+
+
+def test_clear_is_not_undone_by_the_file(tmp_path, monkeypatch):
+    # a file written by another process keeps its entries, but a cleared
+    # process table is not refilled from it
+    p = tmp_path / "shared.tsv"
+    _write(p, ["%s\t1\t1024\t6\t1024" % KEY, "%s\t1\t512\t4\t512" % OTHER])
+    tuning.import_file(p)
+    assert tuning.size() == 2
+    tuning.clear()
+    assert tuning.size() == 0
+    assert tuning.lookup(KEY, tuning.UNORDERED, 1024) is None
+    assert _entries(p)[(KEY, 1, 1024)] == (6, 1024)
