@@ -250,6 +250,21 @@ def bench_uneven(steps, warmup):
     for l in UNEVEN_LOG2N:
         a, b = tuned.class_algo(l)
         plans["N=%d" % (1 << l)] = "%s/%d" % (a.name, b)
+    # each size class of the same mix alone, as its own tuned schedule
+    # (persistent direct launch): where the fused fraction is lost
+    per_class = {}
+    for l in UNEVEN_LOG2N:
+        sel = log2_N[log2_N == l]
+        cj, cin, cf, cout = packed_jobs(sel)
+        cx = DeviceBuffer.from_host(x[:cin], s)  # a class alone packs into fewer words
+        cF = DeviceBuffer((cf,), np.complex128)
+        cy = DeviceBuffer((cout,), np.float64)
+        cs = Schedule(cj, cin, cf, cout, measure=True)
+        cms = time_region(lambda: cs.run_direct(cx, cF, cy, s), s, max(steps, 10), warmup)
+        cN = 1 << int(l)
+        per_class["N=%d" % cN] = {
+            "jobs": int(len(sel)), "us": round(cms * 1e3, 1),
+            "fused_frac": round(24 * cN * len(sel) / (cms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
     return {
         "workload": "%d f64 negacyclic fwd+inv jobs, N in {512..16384} weights %s, %d total "
                     "coefficients" % (UNEVEN_JOBS, UNEVEN_WEIGHTS, total_N),
@@ -263,7 +278,7 @@ def bench_uneven(steps, warmup):
                      "frac": round(bytes_algo / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                      "bytes_per_launch": bytes_algo, "fused_bytes_per_launch": bytes_fused,
                      "fused_frac": round(bytes_fused / (best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
-        "tuned_plans": plans, "autotune_s": round(tune_s, 3),
+        "tuned_plans": plans, "per_class_alone": per_class, "autotune_s": round(tune_s, 3),
         "roundtrip_error_max": err, "schedule": tuned.info()}
 
 
