@@ -358,6 +358,11 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
     return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),
                      "w2048_transform");
+  // Dif16, base_n = 1024, n = 16384: one radix-8 level above chunks of 2048
+  if (p->n == 16384 && p->d_tw_chunk && p->d_tw_base1024 && p->chunk_log2_fwd == 11)
+    return hip_error(cfft_launch::depth16k_transform(fwd, d, batch, stride, p->d_tw,
+                                                     p->d_tw_chunk, p->d_tw_base1024, s),
+                     "depth16k_transform");
   const std::vector<Pass> &global = fwd ? p->fwd_global : p->inv_global;
   if (!global.empty() && global.front().kind != cfft_prog::DEPTH_FWD &&
       global.front().kind != cfft_prog::DEPTH_INV)

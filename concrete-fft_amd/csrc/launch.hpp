@@ -95,6 +95,13 @@ hipError_t depth_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, ui
                          const dc64 *untwist, hipStream_t s);
 hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,
                            const dc64 *tw_blk, hipStream_t s);
+// Dif16 plans with n = 16384 and base_n = 1024 in one pass: both depth
+// levels in registers, 16 base transforms per workgroup (tw: the plan's
+// 16384-point table, tw2: its 2048-point sub-table, tw_blk: the 1024-point
+// r16 block).
+hipError_t depth16k_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                              const dc64 *tw, const dc64 *tw2, const dc64 *tw_blk,
+                              hipStream_t s);
 
 // Register-resident kernels for Dif8 plans with base_n == n == 1024 (r8_exec.hpp).
 bool r8_supported(uint32_t log2n);
