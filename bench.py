@@ -57,14 +57,14 @@ BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
 # HBM bytes per 8192-product launch of the persistent Dif16 kernel, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over 4 launches: FETCH_SIZE x 2
 # (the counter reads half of a plain copy kernel's known read size on this
-# platform) + WRITE_SIZE (exact for a copy): (131390 x 2 + 131106) KiB.  The
+# platform) + WRITE_SIZE (exact for a copy): (131390 x 2 + 131114) KiB.  The
 # figure holds for the kernel sources it was measured on (their SHA-256
 # below); when they change, `traffic` is reported as null until the counters
 # are run again.
-TRAFFIC_BYTES = (131390 * 2 + 131106) * 1024
+TRAFFIC_BYTES = (131390 * 2 + 131114) * 1024
 TRAFFIC_SOURCES = ("concrete-fft_amd/csrc/polymul_kernels.hip", "concrete-fft_amd/csrc/r16_exec.hpp",
                    "concrete-fft_amd/csrc/device_math.hpp")
-TRAFFIC_SHA256 = "c67ee2f84427cf44cfac2052e15cba3b792a37c53ebbefeea6ad03428f97d494"
+TRAFFIC_SHA256 = "bf20ce9430b7b2bb4bcba7044ee69430fd0096677a8d95b684eeac1965c0a710"
 
 
 def traffic_sources_sha():

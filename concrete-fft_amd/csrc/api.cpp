@@ -350,10 +350,14 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
     return hip_error(cfft_launch::wd2_512_transform(fwd, d, batch, stride, p->d_tw,
                                                     p->d_tw_base1024, s),
                      "wd2_512_transform");
-  if ((p->n == 4096 || p->n == 8192) && p->base_n == 1024 && p->d_tw_base1024)  // Dif16
+  if (p->n == 4096 && p->base_n == 1024 && p->d_tw_base1024)  // Dif16
     return hip_error(cfft_launch::depth_transform(fwd, log2n, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),
                      "depth_transform");
+  if (p->n == 8192 && p->base_n == 1024 && p->d_tw_base1024)  // Dif16
+    return hip_error(cfft_launch::depth8k_transform(fwd, d, batch, stride, p->d_tw,
+                                                    p->d_tw_base1024, s),
+                     "depth8k_transform");
   if (p->n == 2048 && p->d_tw_base1024)  // Dif16, base_n = 1024
     return hip_error(cfft_launch::w2048_transform(fwd, d, batch, stride, p->d_tw,
                                                   p->d_tw_base1024, s),
@@ -376,6 +380,10 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
       return hip_error(cfft_launch::w2048_transform(fwd, d, nc, 2048, p->d_tw_chunk,
                                                     p->d_tw_base1024, s),
                        "w2048_transform");
+    if (cl == 13)
+      return hip_error(cfft_launch::depth8k_transform(fwd, d, nc, 8192, p->d_tw_chunk,
+                                                      p->d_tw_base1024, s),
+                       "depth8k_transform");
     return hip_error(cfft_launch::depth_transform(fwd, cl, d, nc, (uint64_t)1 << cl, p->d_tw_chunk,
                                                   p->d_tw_base1024, s),
                      "depth_transform");

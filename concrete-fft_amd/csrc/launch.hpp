@@ -81,9 +81,9 @@ hipError_t w1024_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
 
 // Dif16 plans with n = 2048 and base_n = 1024: one transform per wavefront
 // (r16l_2048_transform); tw is the plan's table, tw_blk its 1024-point block.
-// Dif16 plans with base_n = 1024 and n = 4096 or 8192 (log2n 12, 13): the
-// plan's depth pass in registers around one 1024-point base transform per
-// wave, 8192 points per workgroup (tw: the plan's n-point table, tw_blk: its
+// Dif16 plans with base_n = 1024 and n = 4096 (log2n 12): the plan's depth
+// pass in registers around one 1024-point base transform per wave, two
+// transforms per workgroup (tw: the plan's n-point table, tw_blk: its
 // 1024-point r16 block).
 hipError_t depth_transform(bool fwd, uint32_t log2n, dc64 *data, uint64_t batch, uint64_t stride,
                            const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
@@ -102,6 +102,10 @@ hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
 hipError_t depth16k_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                               const dc64 *tw, const dc64 *tw2, const dc64 *tw_blk,
                               hipStream_t s);
+// Dif16 plans with n = 8192 and base_n = 1024: the plan's radix-8 depth level
+// in registers, two transforms per workgroup through LDS in turn.
+hipError_t depth8k_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                             const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
 
 // Register-resident kernels for Dif8 plans with base_n == n == 1024 (r8_exec.hpp).
 bool r8_supported(uint32_t log2n);

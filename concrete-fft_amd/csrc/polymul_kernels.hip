@@ -306,7 +306,7 @@ k_w2048_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
-// ---- Dif16 plan with base_n = 1024, n = R * 1024 (R = 4, 8) ----
+// ---- Dif16 plan with base_n = 1024, n = R * 1024 (R = 4: n = 4096) ----
 // One transform on R waves, 8 / R transforms per 8-wave workgroup.  Forward:
 // the plan's one depth pass (DEPTH_FWD, radix R: butterfly j < 1024 reads the
 // points k * 1024 + j, its output k, times T[k j], goes to chunk bitrev(k))
@@ -791,33 +791,20 @@ hipError_t w2048_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride
 hipError_t depth_transform(bool fwd, uint32_t log2n, dc64 *data, uint64_t batch, uint64_t stride,
                            const dc64 *tw, const dc64 *tw_blk, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  if (log2n != 12 && log2n != 13) return hipErrorInvalidValue;
+  if (log2n != 12) return hipErrorInvalidValue;
   constexpr size_t lds = (kTwWords + 8 * R16<10>::lds_words) * sizeof(c64);
-  const void *k;
-  if (log2n == 12)
-    k = fwd ? reinterpret_cast<const void *>(k_depth_transform<12, true>)
-            : reinterpret_cast<const void *>(k_depth_transform<12, false>);
-  else
-    k = fwd ? reinterpret_cast<const void *>(k_depth_transform<13, true>)
-            : reinterpret_cast<const void *>(k_depth_transform<13, false>);
+  const void *k = fwd ? reinterpret_cast<const void *>(k_depth_transform<12, true>)
+                      : reinterpret_cast<const void *>(k_depth_transform<12, false>);
   hipError_t e = cfft_launch::ensure_lds(k, lds);
   if (e != hipSuccess) return e;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw);
   const c64 *tb = reinterpret_cast<const c64 *>(tw_blk);
-  const uint64_t per_wg = log2n == 12 ? 2 : 1;
-  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
-  if (log2n == 12) {
-    if (fwd)
-      hipLaunchKernelGGL((k_depth_transform<12, true>), grid, block, lds, s, d, batch, stride, t, tb);
-    else
-      hipLaunchKernelGGL((k_depth_transform<12, false>), grid, block, lds, s, d, batch, stride, t, tb);
-  } else {
-    if (fwd)
-      hipLaunchKernelGGL((k_depth_transform<13, true>), grid, block, lds, s, d, batch, stride, t, tb);
-    else
-      hipLaunchKernelGGL((k_depth_transform<13, false>), grid, block, lds, s, d, batch, stride, t, tb);
-  }
+  const dim3 grid((uint32_t)((batch + 1) / 2)), block(64 * kPW);
+  if (fwd)
+    hipLaunchKernelGGL((k_depth_transform<12, true>), grid, block, lds, s, d, batch, stride, t, tb);
+  else
+    hipLaunchKernelGGL((k_depth_transform<12, false>), grid, block, lds, s, d, batch, stride, t, tb);
   return hipGetLastError();
 }
 

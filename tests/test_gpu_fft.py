@@ -155,7 +155,8 @@ def test_unordered_large_bit_exact(cfft, oracle, n, base_n):
                                                  (1024, 1100, 7, 1024), (1024, 1030, 7, 512),
                                                  (1024, 1024, 32, 512), (2048, 2050, 11, 1024),
                                                  (4096, 4100, 5, 1024), (8192, 8200, 3, 1024),
-                                                 (16384, 16400, 2, 1024), (16384, 16384, 5, 1024)])
+                                                 (16384, 16400, 2, 1024), (16384, 16384, 5, 1024),
+                                                 (8192, 8192, 1, 1024), (8192, 8192, 6, 1024)])
 def test_device_path_with_stride(cfft, oracle, n, stride, batch, base):
     from cfft_amd.device import DeviceBuffer, Stream
     from cfft_amd.unordered import FftAlgo, Method, Plan
