@@ -70,7 +70,8 @@ def build(verbose=False):
     objs = [j[-1] for j in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-lquadmath", "-lpthread"])
+         ["-lquadmath", "-lpthread",
+          "-Wl,--version-script=" + os.path.join(CSRC, "exports.map")])
     os.replace(tmp, LIB)
     # C++ mirror example (links the library through its in-tree rpath)
     host = os.path.join(HERE, "host")

@@ -31,6 +31,20 @@ def test_library_exports_every_declared_symbol():
     assert not [n for n in declared_functions() if n not in cfft_amd.EXPORTED]
 
 
+def test_library_exports_nothing_else():
+    # the dynamic symbol table holds the header's entry points only (the
+    # kernels and host internals stay local to the library)
+    import shutil
+    import subprocess
+    from cfft_amd._lib import LIB_PATH
+    if not shutil.which("nm"):
+        pytest.skip("nm is not available")
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    assert exported == set(declared_functions()), sorted(exported ^ set(declared_functions()))
+
+
 def test_abi_version():
     import cfft_amd
     assert cfft_amd.lib.cfft_abi_version() == 1
