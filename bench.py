@@ -57,14 +57,14 @@ BYTES_PER_PRODUCT = 24 * N  # A 8N + B 8N read, C 8N written
 # HBM bytes per 8192-product launch of the persistent Dif16 kernel, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over 4 launches: FETCH_SIZE x 2
 # (the counter reads half of a plain copy kernel's known read size on this
-# platform) + WRITE_SIZE (exact for a copy): (131390 x 2 + 131114) KiB.  The
+# platform) + WRITE_SIZE (exact for a copy): (131391 x 2 + 131113) KiB.  The
 # figure holds for the kernel sources it was measured on (their SHA-256
 # below); when they change, `traffic` is reported as null until the counters
 # are run again.
-TRAFFIC_BYTES = (131390 * 2 + 131114) * 1024
-TRAFFIC_SOURCES = ("concrete-fft_amd/csrc/polymul_kernels.hip", "concrete-fft_amd/csrc/r16_exec.hpp",
+TRAFFIC_BYTES = (131391 * 2 + 131113) * 1024
+TRAFFIC_SOURCES = ("concrete-fft_amd/csrc/polymul_n2048_kernels.hip", "concrete-fft_amd/csrc/r16_exec.hpp",
                    "concrete-fft_amd/csrc/device_math.hpp")
-TRAFFIC_SHA256 = "bf20ce9430b7b2bb4bcba7044ee69430fd0096677a8d95b684eeac1965c0a710"
+TRAFFIC_SHA256 = "ac50f118ad451f083dbf608844c3896354b4151d9029465d44a98b7a9a635fa4"
 
 
 def traffic_sources_sha():

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
 HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-munsafe-fp-atomics"]
 
-HIP_SOURCES = ["fft_kernels.hip", "polymul_kernels.hip", "fft128_kernels.hip",
+HIP_SOURCES = ["fft_kernels.hip", "polymul_kernels.hip", "polymul_n2048_kernels.hip", "fft128_kernels.hip",
                "exact_kernels.hip", "extprod_kernels.hip", "uneven_kernels.hip", "noalloc.hip", "serve_kernels.hip",
                "large_transform_kernels.hip"]
 HOST_HIP_SOURCES = ["api.cpp", "fft128_api.cpp", "products_api.cpp", "uneven_api.cpp",
