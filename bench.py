@@ -32,8 +32,10 @@ transforms of 32-bit limbs, with both results checked equal and a sample
 checked against the u64 schoolbook.  `ordered` times the ordered c64 plan at
 n = 1024, batch 1 (fwd and inv per call) and reports its error against the
 oracle and its round trip.  `transforms` reports batched unordered forward
-transforms (n = 1024 x 8192 and n = 2048 x 4096) per second for several plans.  `--mode copy`
-runs one
+transforms (n = 1024 x 8192 and n = 2048 x 4096) per second for several plans.  `matrix`
+restates the reference's bench matrix (unordered fwd / inv / fwd_monomial at n = 2^8 ... 2^16,
+fft128 at n = 64 ... 16384) and sweeps the fused product over N = 512 ... 16384 (also
+`--mode products`).  `--mode copy` runs one
 device-to-device copy of --copy-mib MiB (the reference kernel for scaling HBM
 byte counters).
 """
@@ -549,7 +551,7 @@ def bench_matrix(steps=100):
     import cfft_amd
     from cfft_amd import check, fft128, lib, ordered
     from cfft_amd.device import DeviceBuffer, Stream
-    from cfft_amd.unordered import Method, Plan
+    from cfft_amd.unordered import FftAlgo, Method, Plan
     s = Stream()
     rng = np.random.default_rng(9)
     big = DeviceBuffer.from_host(rng.uniform(-1, 1, 1 << 24).view(np.complex128), s)
@@ -604,8 +606,36 @@ def bench_matrix(steps=100):
                                                     / PEAK_HBM_GBS, 4)
         r128[str(n)] = row
     return {"workload": "reference bench matrix on device buffers (batch-1 device time per "
-                        "call; batched: 8 Mi c64 / 4 Mi f128 complex points per call)",
-            "unordered": rows, "fft128": r128}
+                        "call; batched: 8 Mi c64 / 4 Mi f128 complex points per call); fused "
+                        "products: 16 Mi coefficients per operand per call",
+            "unordered": rows, "fft128": r128, "products": bench_products()}
+
+
+def bench_products(steps=20):
+    """The fused negacyclic product across N = 512 ... 16384 (the caller-side
+    path of src/unordered.rs:826-940 at other sizes): Dif16 plans with base_n =
+    min(N/2, 1024), batch = 16 Mi / N (each operand 128 MiB), products/s and
+    the fraction of 8 TB/s at 24 N bytes per product."""
+    from cfft_amd import check, lib
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    s = Stream()
+    rng = np.random.default_rng(10)
+    prods = {}
+    for lg in range(9, 15):
+        N = 1 << lg
+        plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, min(N // 2, 1024)))
+        batch = (1 << 24) // N
+        a = DeviceBuffer.from_host(rng.integers(0, 1 << 64, batch * N, dtype=np.uint64), s)
+        b = DeviceBuffer.from_host(rng.integers(-128, 128, batch * N, dtype=np.int64), s)
+        c = DeviceBuffer((batch * N,), np.uint64)
+        msb = time_region(lambda: check(lib.cfft_negacyclic_polymul_u64(
+            plan.handle, a.dptr, b.dptr, c.dptr, batch, s.handle)), s, steps, 3)
+        prods[str(N)] = {"batch": batch, "ms": round(msb, 4),
+                         "products_per_s": round(batch / (msb * 1e-3), 1),
+                         "hbm_frac": round(24 * N * batch / (msb * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        del a, b, c
+    return prods
 
 
 def copy_probe(mib):
@@ -633,7 +663,7 @@ def main():
                     help="products timed in the one-call-per-polynomial form")
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
-    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered",
+    ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered", "products",
                                        "transforms", "matrix", "host", "copy"], default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
@@ -677,6 +707,9 @@ def main():
         return 0
     if args.mode == "matrix":
         print(json.dumps({"mode": "matrix", "matrix": bench_matrix()}))
+        return 0
+    if args.mode == "products":
+        print(json.dumps({"mode": "products", "products": bench_products()}))
         return 0
     if args.mode == "host":
         print(json.dumps({"mode": "host", "host_path": bench_host_path()}))

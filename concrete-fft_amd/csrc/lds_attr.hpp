@@ -1,4 +1,6 @@
-// lds_attr.hpp -- raise a kernel's dynamic-LDS limit above the 64 KiB default,
+// lds_attr.hpp -- launch helpers shared by the kernel translation units.
+//
+// ensure_lds: raise a kernel's dynamic-LDS limit above the 64 KiB default,
 // per (device, kernel), to the largest size requested so far (not to the
 // device maximum: a kernel's static LDS counts against the same budget).
 // Thread-safe; host code shared by the kernel translation units.
@@ -29,6 +31,27 @@ inline hipError_t ensure_lds(const void *kernel, size_t bytes) {
   e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) return e;
   set_to[{dev, kernel}] = bytes;
+  return hipSuccess;
+}
+
+// Compute units of the current device (read once per device), for the grids
+// of persistent kernels.
+inline hipError_t cu_count(int *cus) {
+  static std::mutex mu;
+  static std::map<int, int> of;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  auto it = of.find(dev);
+  if (it == of.end()) {
+    int v = 0;
+    e = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    if (v <= 0) return hipErrorInvalidDevice;
+    it = of.emplace(dev, v).first;
+  }
+  *cus = it->second;
   return hipSuccess;
 }
 

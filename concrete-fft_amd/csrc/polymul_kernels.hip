@@ -265,22 +265,46 @@ k_depth_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
 // and torus rounding follow before the stores.  The same operations in the
 // same order as the generic fused product (k_polymul_generic), so the same
 // bits.
+
+// One operand's raw depth-pass inputs for one thread: point (i, k) is
+// q = k * 1024 + t + TJ i, held as the two u64 words at q and q + n.
 template <int LOG2N>
-__device__ __forceinline__ void depth_pass_to_lds(c64 *X, uint32_t t, const c64 *tw,
-                                                  const c64 *twist, const uint64_t *a,
-                                                  const int64_t *b) {
-  constexpr uint32_t n = 1u << LOG2N;
-  constexpr int R = 1 << (LOG2N - 10);
-  constexpr uint32_t TJ = n / 16, NBD = 16 / R, CH = R16<10>::lds_words;
-#pragma unroll 1
-  for (uint32_t i = 0; i < NBD; ++i) {
+struct DepthRaw {
+  static constexpr int R = 1 << (LOG2N - 10);
+  static constexpr uint32_t NBD = 16 / R;
+  uint64_t re[NBD][R], im[NBD][R];
+};
+
+template <int LOG2N, class T>
+__device__ __forceinline__ void depth_raw_load(DepthRaw<LOG2N> &d, const T *x, uint32_t t) {
+  constexpr uint32_t n = 1u << LOG2N, TJ = n / 16;
+#pragma unroll
+  for (uint32_t i = 0; i < DepthRaw<LOG2N>::NBD; ++i)
+#pragma unroll
+    for (int k = 0; k < DepthRaw<LOG2N>::R; ++k) {
+      const uint32_t q = (uint32_t)k * 1024 + t + TJ * i;
+      d.re[i][k] = (uint64_t)ld_stream(&x[q]);
+      d.im[i][k] = (uint64_t)ld_stream(&x[q + n]);
+    }
+}
+
+// Fold + twist + the plan's depth pass (radix R, as k_depth_transform) on
+// inputs already in registers (TORUS: A's u64 torus words, else B's i64
+// digits), chunks to LDS
+template <int LOG2N, bool TORUS>
+__device__ __forceinline__ void depth_raw_to_lds(const DepthRaw<LOG2N> &d, c64 *X, uint32_t t,
+                                                 const c64 *tw, const c64 *twist) {
+  constexpr uint32_t n = 1u << LOG2N, TJ = n / 16, CH = R16<10>::lds_words;
+  constexpr int R = DepthRaw<LOG2N>::R;
+#pragma unroll
+  for (uint32_t i = 0; i < DepthRaw<LOG2N>::NBD; ++i) {
     const uint32_t j = t + TJ * i;
     c64 v[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const uint32_t q = (uint32_t)k * 1024 + j;
-      const c64 x = a ? mk(torus_to_f64(ld_stream(&a[q])), torus_to_f64(ld_stream(&a[q + n])))
-                      : mk(i64_to_f64(ld_stream(&b[q])), i64_to_f64(ld_stream(&b[q + n])));
+      const c64 x = TORUS ? mk(torus_to_f64(d.re[i][k]), torus_to_f64(d.im[i][k]))
+                          : mk(i64_to_f64((int64_t)d.re[i][k]), i64_to_f64((int64_t)d.im[i][k]));
       v[k] = cmul(twist[q], x);
     }
     butterfly<R, true>(v);
@@ -288,10 +312,17 @@ __device__ __forceinline__ void depth_pass_to_lds(c64 *X, uint32_t t, const c64 
     for (int k = 1; k < R; ++k) v[k] = cmul(tw[(uint32_t)k * j], v[k]);
 #pragma unroll
     for (int k = 0; k < R; ++k) X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)] = v[k];
+    sched_fence();  // one butterfly's twist and twiddle loads in flight at a time
   }
 }
 
-template <int LOG2N>
+// Persistent form: each workgroup loops over its share of the batch, G
+// products at a time.  PRE_B: B's inputs are loaded while A's base transform
+// runs; PRE_A: the next round's A inputs are loaded while the inverse base
+// transform runs (so only the first round waits for A).  A group past the
+// batch works on the workgroup's first product of that round and stores
+// nothing; every wave runs every round and reaches every barrier.
+template <int LOG2N, bool PRE_B, bool PRE_A>
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_depth_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
                 uint64_t *__restrict__ C, uint64_t batch, const c64 *__restrict__ tw,
@@ -301,69 +332,97 @@ k_depth_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   constexpr uint32_t n = 1u << LOG2N;
   constexpr int R = 1 << (LOG2N - 10);
   constexpr uint32_t TJ = n / 16;        // threads per product
-  constexpr uint32_t G = 64 * kPW / TJ;  // products per workgroup
+  constexpr uint32_t G = 64 * kPW / TJ;  // products per workgroup and round
   constexpr uint32_t NBD = 16 / R;
   constexpr uint32_t CH = S::lds_words;
   extern __shared__ c64 smem[];
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
-  const uint32_t g = threadIdx.x / TJ, t = threadIdx.x % TJ;
-  const uint64_t pr = (uint64_t)blockIdx.x * G + g;
-  // a group past the batch works on the workgroup's first product and stores
-  // nothing; every wave reaches every barrier
-  const bool active = pr < batch;
-  const uint64_t off = (active ? pr : (uint64_t)blockIdx.x * G) * (2ull * n);
-  c64 *X = smem + kTwWords + g * (R * CH);
-  const uint32_t c = t >> 6, l = t & 63;
-  c64 *xc = X + c * CH;
+  const uint64_t step = (uint64_t)gridDim.x * G;
+  // round r of this workgroup: products blockIdx.x * G + r * step + g
+  auto offset_of = [&](uint64_t first, uint32_t g) -> uint64_t {
+    const uint64_t pr = first + g;
+    return (pr < batch ? pr : first) * (2ull * n);
+  };
+  uint64_t first = (uint64_t)blockIdx.x * G;
+  DepthRaw<LOG2N> ra;
+  if (PRE_A) depth_raw_load<LOG2N>(ra, A + offset_of(first, threadIdx.x / TJ), threadIdx.x % TJ);
   __syncthreads();  // twiddle block staged
-  depth_pass_to_lds<LOG2N>(X, t, tw, twist, A + off, nullptr);
-  __syncthreads();
-  c64 fa[16];
+  while (first < batch) {
+    // recomputed per round (opaque) so the per-lane addresses below are not
+    // hoisted out of the loop and kept live across it
+    const uint32_t tid = launder_u32(threadIdx.x);
+    const uint32_t g = tid / TJ, t = tid % TJ;
+    c64 *X = smem + kTwWords + g * (R * CH);
+    const uint32_t c = t >> 6, l = t & 63;
+    c64 *xc = X + c * CH;
+    const bool active = first + g < batch;
+    const uint64_t off = offset_of(first, g);
+    if (!PRE_A) depth_raw_load<LOG2N>(ra, A + off, t);
+    depth_raw_to_lds<LOG2N, true>(ra, X, t, tw, twist);
+    __syncthreads();
+    c64 fa[16];
+    {
+      DepthRaw<LOG2N> rb;
+      if (PRE_B) {
+        depth_raw_load<LOG2N>(rb, B + off, t);
+        sched_fence();
+      }
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) fa[m] = xc[r16_pad(l + 64 * m)];
-  r16l_transform<10, true>(fa, fa, xc, l, ltw);
-  __syncthreads();  // every chunk read before B's depth pass overwrites it
-  depth_pass_to_lds<LOG2N>(X, t, tw, twist, nullptr, B + off);
-  __syncthreads();
-  {
-    c64 fb[16];
-#pragma unroll
-    for (uint32_t m = 0; m < 16; ++m) fb[m] = xc[r16_pad(l + 64 * m)];
-    r16l_transform<10, true>(fb, fb, xc, l, ltw);
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], fb[s]);
-  }
-  c64 u[16];
-  r16_end_to_input<10>(fa, u);
-  r16l_transform<10, false>(u, u, xc, l, ltw);
-  wave_lds_fence();
-#pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
-  __syncthreads();
-  uint64_t *cp = C + off;
-#pragma unroll 1
-  for (uint32_t i = 0; i < NBD; ++i) {
-    const uint32_t j = t + TJ * i;
-    c64 w[R];
-#pragma unroll
-    for (int k = 0; k < R; ++k) w[k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
-#pragma unroll
-    for (int k = 1; k < R; ++k) {
-      c64 f = tw[(uint32_t)k * j];
-      f.im = -f.im;
-      w[k] = cmul(f, w[k]);
+      for (uint32_t m = 0; m < 16; ++m) fa[m] = xc[r16_pad(l + 64 * m)];
+      r16l_transform<10, true>(fa, fa, xc, l, ltw);
+      __syncthreads();  // every chunk read before B's depth pass overwrites it
+      if (!PRE_B) depth_raw_load<LOG2N>(rb, B + off, t);
+      depth_raw_to_lds<LOG2N, false>(rb, X, t, tw, twist);
     }
-    butterfly<R, false>(w);
-    if (active) {
+    __syncthreads();
+    {
+      c64 fb[16];
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const uint32_t q = (uint32_t)k * 1024 + j;
-        const c64 r = cmul(untwist[q], w[k]);
-        st_stream(from_torus(r.re), &cp[q]);
-        st_stream(from_torus(r.im), &cp[q + n]);
+      for (uint32_t m = 0; m < 16; ++m) fb[m] = xc[r16_pad(l + 64 * m)];
+      r16l_transform<10, true>(fb, fb, xc, l, ltw);
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], fb[s]);
+    }
+    const uint64_t next = first + step;
+    if (PRE_A) {
+      // the next round's A (past the batch: this round's, never used)
+      depth_raw_load<LOG2N>(ra, A + offset_of(next < batch ? next : first, g), t);
+      sched_fence();
+    }
+    c64 u[16];
+    r16_end_to_input<10>(fa, u);
+    r16l_transform<10, false>(u, u, xc, l, ltw);
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) xc[r16_pad(l + 64 * S::end_m(s))] = u[s];
+    __syncthreads();
+    uint64_t *cp = C + off;
+#pragma unroll 1
+    for (uint32_t i = 0; i < NBD; ++i) {
+      const uint32_t j = t + TJ * i;
+      c64 w[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) w[k] = X[(uint32_t)bitrev_digit<R>(k) * CH + r16_pad(j)];
+#pragma unroll
+      for (int k = 1; k < R; ++k) {
+        c64 f = tw[(uint32_t)k * j];
+        f.im = -f.im;
+        w[k] = cmul(f, w[k]);
+      }
+      butterfly<R, false>(w);
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const uint32_t q = (uint32_t)k * 1024 + j;
+          const c64 r = cmul(untwist[q], w[k]);
+          st_stream(from_torus(r.re), &cp[q]);
+          st_stream(from_torus(r.im), &cp[q + n]);
+        }
       }
     }
+    __syncthreads();  // every chunk read before the next round's depth pass
+    first = next;
   }
 }
 
@@ -673,21 +732,32 @@ hipError_t depth_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, ui
   if (batch == 0) return hipSuccess;
   if (log2n < 11 || log2n > 13) return hipErrorInvalidValue;
   constexpr size_t lds = (kTwWords + 8 * R16<10>::lds_words) * sizeof(c64);
-  const void *k = log2n == 11   ? reinterpret_cast<const void *>(k_depth_polymul<11>)
-                  : log2n == 12 ? reinterpret_cast<const void *>(k_depth_polymul<12>)
-                                : reinterpret_cast<const void *>(k_depth_polymul<13>);
+  // both operands loaded a step ahead at N = 4096 and 8192; at N = 16384
+  // (8 points per depth butterfly) B's early load would spill, so only the
+  // next round's A is (measured: 38.0 / 18.1 / 7.0 M products/s against
+  // 31.6 / 14.1 / 6.1 M with every load at its depth pass)
+  const void *k = log2n == 11   ? reinterpret_cast<const void *>(k_depth_polymul<11, true, true>)
+                  : log2n == 12 ? reinterpret_cast<const void *>(k_depth_polymul<12, true, true>)
+                                : reinterpret_cast<const void *>(k_depth_polymul<13, false, true>);
   hipError_t e = cfft_launch::ensure_lds(k, lds);
+  if (e != hipSuccess) return e;
+  int cus = 0;
+  e = cfft_launch::cu_count(&cus);
   if (e != hipSuccess) return e;
   const c64 *t = reinterpret_cast<const c64 *>(tw), *tb = reinterpret_cast<const c64 *>(tw_blk);
   const c64 *z = reinterpret_cast<const c64 *>(twist), *u = reinterpret_cast<const c64 *>(untwist);
-  const uint64_t per_wg = 8u >> (log2n - 10);  // 8 / R products
-  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
+  const uint64_t per_wg = 8u >> (log2n - 10);  // 8 / R products per round
+  const uint64_t want = (batch + per_wg - 1) / per_wg;
+  const dim3 grid((uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus)), block(64 * kPW);
   if (log2n == 11)
-    hipLaunchKernelGGL(k_depth_polymul<11>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
+    hipLaunchKernelGGL((k_depth_polymul<11, true, true>), grid, block, lds, s, a, b, c, batch, t,
+                       tb, z, u);
   else if (log2n == 12)
-    hipLaunchKernelGGL(k_depth_polymul<12>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
+    hipLaunchKernelGGL((k_depth_polymul<12, true, true>), grid, block, lds, s, a, b, c, batch, t,
+                       tb, z, u);
   else
-    hipLaunchKernelGGL(k_depth_polymul<13>, grid, block, lds, s, a, b, c, batch, t, tb, z, u);
+    hipLaunchKernelGGL((k_depth_polymul<13, false, true>), grid, block, lds, s, a, b, c, batch, t,
+                       tb, z, u);
   return hipGetLastError();
 }
 

@@ -143,3 +143,20 @@ def test_persistent_product_batches(cfft, oracle, batch):
     c = products.negacyclic_polymul_u64_host(plan, a, b)
     ref = oracle.polymul_f64(a, b, 6, 1024, threads=16)
     assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
+
+
+@pytest.mark.parametrize("N,batch", [(4096, 1), (4096, 1030), (8192, 3), (8192, 515),
+                                     (16384, 2), (16384, 259)])
+def test_depth_product_rounds(cfft, oracle, N, batch):
+    """N = 4096, 8192, 16384 on Dif16 / 1024 plans (the persistent depth-pass
+    product: 8 / R products per workgroup and round, at most one workgroup
+    per CU, operands of the next step loaded during the current one):
+    bit-exact against the oracle for one round, and for batches one round
+    past a full grid of 256 workgroups."""
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    a, b = rand_inputs(N + batch, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, 6, 1024, threads=16)
+    assert np.array_equal(c, ref), (N, batch, int(np.sum(c != ref)))
