@@ -24,24 +24,38 @@ using cfft_prog::Program;
 
 namespace {
 
-__device__ __forceinline__ int64_t gadget_digit(uint64_t x, uint32_t level, uint32_t levels,
-                                                uint32_t base_log) {
+// One level of the gadget decomposition, without branches or loops: the
+// balanced digits of the rounded value st (rep = levels * base_log bits) are
+// the plain base-2^B digits of st + (half in every digit position), each
+// minus half -- the carry loop of the decomposition done by one addition
+// (the same digits; the wrap-around past rep bits is the top digit's carry,
+// which the decomposition drops as well).
+struct Gadget {
+  uint32_t shift;   // 64 - rep: the rounding position (0: no rounding)
+  uint32_t down;    // base_log * (levels - 1 - level): this level's digit position
+  uint64_t keep;    // the low rep bits
+  uint64_t offset;  // half in every digit position
+  uint64_t mask;    // one digit
+  int64_t half;
+};
+
+__device__ __forceinline__ Gadget gadget_of(uint32_t level, uint32_t levels, uint32_t base_log) {
+  Gadget g;
   const uint32_t rep = levels * base_log;
-  const uint32_t shift = 64 - rep;
-  uint64_t st = shift == 0 ? x : (x >> shift) + ((x >> (shift - 1)) & 1u);
-  if (rep < 64) st &= ((uint64_t)1 << rep) - 1;
-  const uint64_t mask = ((uint64_t)1 << base_log) - 1;
-  const int64_t half = (int64_t)1 << (base_log - 1);
-  int64_t d = 0;
-  for (int32_t i = (int32_t)levels - 1; i >= (int32_t)level; --i) {
-    d = (int64_t)(st & mask);
-    st >>= base_log;
-    if (d >= half) {
-      d -= (int64_t)1 << base_log;
-      st += 1;
-    }
-  }
-  return d;
+  g.shift = 64 - rep;
+  g.down = base_log * (levels - 1 - level);
+  g.keep = rep < 64 ? ((uint64_t)1 << rep) - 1 : ~(uint64_t)0;
+  g.mask = ((uint64_t)1 << base_log) - 1;
+  g.half = (int64_t)1 << (base_log - 1);
+  g.offset = 0;
+  for (uint32_t i = 0; i < levels; ++i) g.offset = (g.offset << base_log) | (uint64_t)g.half;
+  return g;
+}
+
+__device__ __forceinline__ int64_t gadget_digit(const Gadget &g, uint64_t x) {
+  const uint64_t up = g.shift ? (x >> (g.shift - 1)) & 1u : 0;  // ties round up
+  const uint64_t st = ((x >> g.shift) + up) & g.keep;
+  return (int64_t)(((st + g.offset) >> g.down) & g.mask) - g.half;
 }
 
 // key: npolys u64 torus polynomials -> Fourier (plan order), one per transform
@@ -90,13 +104,12 @@ __global__ void __launch_bounds__(512) k_extprod_rows(const uint64_t *__restrict
   const uint64_t t = (uint64_t)blockIdx.x * G + g;  // (p * k1 + poly) * levels + lvl
   const bool active = t < nblocks;
   c64 *lds = smem + g * lds_words(n);
-  const uint32_t lvl = (uint32_t)(t % levels);
+  const Gadget gd = gadget_of((uint32_t)(t % levels), levels, base_log);
   const uint64_t *src = glwe + (t / levels) * 2ull * n;
   auto load = [&](uint32_t j) -> c64 {
     if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j],
-                mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
-                   (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
+    return cmul(twist[j], mk((double)(int32_t)gadget_digit(gd, ld_stream(&src[j])),
+                             (double)(int32_t)gadget_digit(gd, ld_stream(&src[j + n]))));
   };
   c64 *dst = S + t * n;
   auto store = [&](uint32_t pos, c64 v) {
@@ -202,16 +215,15 @@ k_extprod_rows_w(const uint64_t *__restrict__ glwe, const uint64_t *__restrict__
   const uint64_t t = t0 + half;  // (p k1 + poly) levels + lvl
   const bool active = t < nrows;
   c64 *x = smem + kETw + wave * R::lds_words + half * R16<(LOG2N <= 10 ? LOG2N : 10)>::lds_words;
-  const uint32_t lvl = (uint32_t)(t % levels);
+  const Gadget gd = gadget_of((uint32_t)(t % levels), levels, base_log);
   const uint64_t q = active ? t / levels : 0;  // polynomial p k1 + poly
   const uint64_t *src =
       glwe_tab ? reinterpret_cast<const uint64_t *>(glwe_tab[q / k1]) + (q % k1) * 2ull * n
                : glwe + q * 2ull * n;
   c64 *dst = S + t * n;
   auto digit_pt = [&](uint32_t j) -> c64 {
-    return cmul(twist[j],
-                mk((double)(int32_t)gadget_digit(ld_stream(&src[j]), lvl, levels, base_log),
-                   (double)(int32_t)gadget_digit(ld_stream(&src[j + n]), lvl, levels, base_log)));
+    return cmul(twist[j], mk((double)(int32_t)gadget_digit(gd, ld_stream(&src[j])),
+                             (double)(int32_t)gadget_digit(gd, ld_stream(&src[j + n]))));
   };
   if constexpr (LOG2N <= 10) {
     using L = R16<LOG2N>;
@@ -243,8 +255,8 @@ k_extprod_rows_w(const uint64_t *__restrict__ glwe, const uint64_t *__restrict__
 #pragma unroll
         for (uint32_t m = m0; m < m0 + 4; ++m)
           v[m] = cmul(twist[h * 1024 + l + 64 * m],
-                      mk((double)(int32_t)gadget_digit(r[2 * m], lvl, levels, base_log),
-                         (double)(int32_t)gadget_digit(r[2 * m + 1], lvl, levels, base_log)));
+                      mk((double)(int32_t)gadget_digit(gd, r[2 * m]),
+                         (double)(int32_t)gadget_digit(gd, r[2 * m + 1])));
         sched_fence();
       }
     }
@@ -292,25 +304,31 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   };
   // Fourier multiply-accumulate row by row: each row's key and spectrum
   // points of the lane are loaded together (one memory round trip per row,
-  // not per point); every point still sums its rows in ascending order
+  // not per point); every point still sums its rows in ascending order.  At
+  // n = 2048 the second half accumulates beside the first, so its rows are
+  // loaded 8 points at a time (16 would spill).
+  constexpr uint32_t MC = LOG2N > 10 ? 8 : 16;
   auto mac_rows = [&](c64 *acc, uint32_t base) {
 #pragma unroll
     for (uint32_t m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
     for (uint32_t r = 0; r < rows; ++r) {
       const c64 *kr = keyf + ((uint64_t)r * k1 + c) * n + base + l;
       const c64 *sr = sp + (uint64_t)r * n + base + l;
-      c64 kv[16], sv[16];
 #pragma unroll
-      for (uint32_t m = 0; m < 16; ++m) {
-        kv[m] = kr[T * m];
-        sv[m] = sr[T * m];
-      }
+      for (uint32_t m0 = 0; m0 < 16; m0 += MC) {
+        c64 kv[MC], sv[MC];
 #pragma unroll
-      for (uint32_t m = 0; m < 16; ++m) {
-        const c64 pr = cmul(kv[m], sv[m]);
-        acc[m] = mk(acc[m].re + pr.re, acc[m].im + pr.im);
+        for (uint32_t m = 0; m < MC; ++m) {
+          kv[m] = kr[T * (m0 + m)];
+          sv[m] = sr[T * (m0 + m)];
+        }
+#pragma unroll
+        for (uint32_t m = 0; m < MC; ++m) {
+          const c64 pr = cmul(kv[m], sv[m]);
+          acc[m0 + m] = mk(acc[m0 + m].re + pr.re, acc[m0 + m].im + pr.im);
+        }
+        sched_fence();  // one row's (or half row's) loads in flight at a time
       }
-      sched_fence();  // one row's loads in flight at a time
     }
   };
   if constexpr (LOG2N <= 10) {
