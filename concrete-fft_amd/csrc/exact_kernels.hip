@@ -14,6 +14,8 @@
 // Kernel 2: per product and weight k, the inverse program's first pass reads
 // sum_{i+j=k} F[a_i] F[b_j] (i ascending), the last pass untwists, rounds to
 // an integer and accumulates 2^(16 k) D_k into C (owned by one workgroup).
+// Dif16 plans at N = 2048 and 4096 have register-resident forms below (one
+// fused kernel at N = 2048).
 #include <hip/hip_runtime.h>
 
 #include "lds_attr.hpp"
@@ -137,38 +139,41 @@ __global__ void __launch_bounds__(512) k_exact_inv(const c64 *__restrict__ F,
 }
 
 // ---- register-resident form for Dif16 plans with n = 1024 (N = 2048) ----
-// Forward: one wave per (product, limb index i) transforms a_i and b_i
-// together (wave-local Dif16 schedule, twiddles shared); 8 waves per
-// workgroup (two products) share the twiddle block staged in LDS.  The
-// product's four waves then meet in their exchange buffers, half of the
-// slots at a time, and each lane forms the four weight spectra
+// One fused kernel, two products per 8-wave workgroup sharing the twiddle
+// block staged in LDS.  Forward: one wave per (product, limb index i)
+// transforms a_i and b_i together (wave-local Dif16 schedule, twiddles
+// shared).  The product's four waves then meet in their exchange buffers,
+// half of the slots at a time, and each lane forms the four weight spectra
 // W_k = sum_{i<=k} F[a_i] F[b_{k-i}] (i ascending) of one point of each
-// product: the scratch holds 4 spectra per product, written once and read
-// once by the inverse.
+// product, keeping them (two halves x two products x four weights) in
+// registers until every limb spectrum is read; then it writes them into the
+// exchange buffers, W_k of product pp into wave 4 pp + k's buffer at its
+// point index.  Inverse: wave 4 pp + k transforms W_k back from LDS, rounds,
+// shifts by 16 k and leaves its parts in its buffer; after a barrier the
+// product's first wave sums the four buffers mod 2^64.  No scratch.
 constexpr uint32_t kXW = 8;
 constexpr uint32_t kXTw = 16 * 64 + 16 * 4;
 constexpr size_t kXLds = (kXTw + kXW * R16<10>::lds_words) * sizeof(c64);
 
 __global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
-                c64 *__restrict__ F, double *margin, uint64_t batch,
-                const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
+k_exact_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
+            uint64_t *__restrict__ C, double *margin, uint64_t batch,
+            const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
   using S = R16<10>;
   constexpr uint32_t n = S::n;
   extern __shared__ c64 smem[];
   c64 *ltw = smem;
   for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_r16[i];
+  double *s_worst = reinterpret_cast<double *>(smem + kXTw + kXW * S::lds_words);
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t u = (uint64_t)blockIdx.x * kXW + wave;  // 4 p + i
-  const bool active = u < batch * 4;  // every wave reaches the barriers below
+  const uint32_t pp = wave >> 2, li = wave & 3;  // product in the workgroup; limb, then weight
+  const uint64_t p = (uint64_t)blockIdx.x * 2 + pp;
+  const bool active = p < batch;  // every wave reaches the barriers below
   c64 *x = smem + kXTw + wave * S::lds_words;
   const uint32_t l = threadIdx.x & 63u;
-  const uint64_t p = u >> 2;
-  const uint32_t li = (uint32_t)(u & 3);
   c64 fa[16], fb[16];
   if (active) {
-    if (li == 0 && l == 0 && margin) margin[p] = 0.0;
     const uint64_t *a = A + p * (2 * n);
     const uint64_t *b = B + p * (2 * n);
 #pragma unroll
@@ -182,15 +187,13 @@ k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
     }
     r16w_transform<true, 2>(fa, fb, x, l, ltw);
   }
-  // weights: the buffer of wave 4 pp + i holds, for the half's slots s,
-  // a_i's slot s of lane l at (s - 8 h) * 64 + l and b_i's 512 words later;
-  // thread t forms the four weights of point t of the half of both products
   const c64 *xs = smem + kXTw;
   const uint32_t t = threadIdx.x;
+  c64 wq[2][2][4];  // [half][product][weight] at this thread's point of the half
 #pragma unroll
   for (uint32_t h = 0; h < 2; ++h) {
     wave_lds_fence();
-    if (active) {  // (an inactive wave's buffer belongs to no product)
+    if (active) {
 #pragma unroll
       for (uint32_t s = 0; s < 8; ++s) {
         x[s * 64 + l] = fa[8 * h + s];
@@ -198,74 +201,64 @@ k_exact_fwd_r16(const uint64_t *__restrict__ A, const uint64_t *__restrict__ B,
       }
     }
     __syncthreads();
-    const uint32_t q = (t & 63) + S::T * S::end_m(8 * h + (t >> 6));
 #pragma unroll
-    for (uint32_t pp = 0; pp < 2; ++pp) {
-      const uint64_t pq = (uint64_t)blockIdx.x * 2 + pp;
-      if (pq < batch) {
-        c64 wa[4], wb[4];
+    for (uint32_t q2 = 0; q2 < 2; ++q2) {
+      c64 wa[4], wb[4];
 #pragma unroll
-        for (uint32_t i = 0; i < 4; ++i) {
-          wa[i] = xs[(4 * pp + i) * S::lds_words + t];
-          wb[i] = xs[(4 * pp + i) * S::lds_words + 512 + t];
-        }
-        c64 *W = F + pq * 4ull * n;
+      for (uint32_t i = 0; i < 4; ++i) {
+        wa[i] = xs[(4 * q2 + i) * S::lds_words + t];
+        wb[i] = xs[(4 * q2 + i) * S::lds_words + 512 + t];
+      }
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
-          c64 d = cmul(wa[0], wb[k]);
+      for (uint32_t k = 0; k < 4; ++k) {
+        c64 d = cmul(wa[0], wb[k]);
 #pragma unroll
-          for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(wa[i], wb[k - i]));
-          st_stream(d, &W[k * n + q]);
-        }
+        for (uint32_t i = 1; i <= k; ++i) d = cadd(d, cmul(wa[i], wb[k - i]));
+        wq[h][q2][k] = d;
       }
     }
-    __syncthreads();  // the buffers are rewritten for the second half
+    __syncthreads();  // every limb spectrum of the half read
   }
-}
-
-// Inverse: 4 waves per product, wave k computes 2^(16 k) D_k (inverse
-// transform of W_k, rounding, shift) into its own LDS buffer;
-// after a barrier the product's first wave sums the four buffers mod 2^64.
-__global__ void __launch_bounds__(64 * kXW) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *margin,
-                uint64_t batch, const c64 *__restrict__ tw_r16, const c64 *__restrict__ twist) {
-  using S = R16<10>;
-  constexpr uint32_t n = S::n;
-  extern __shared__ c64 smem[];
-  c64 *ltw = smem;
-  for (uint32_t i = threadIdx.x; i < kXTw; i += blockDim.x) ltw[i] = tw_r16[i];
+  // W_k of product q2, point q, to wave 4 q2 + k's buffer at q
+#pragma unroll
+  for (uint32_t h = 0; h < 2; ++h) {
+    const uint32_t q = (t & 63) + S::T * S::end_m(8 * h + (t >> 6));
+#pragma unroll
+    for (uint32_t q2 = 0; q2 < 2; ++q2)
+#pragma unroll
+      for (uint32_t k = 0; k < 4; ++k) smem[kXTw + (4 * q2 + k) * S::lds_words + q] = wq[h][q2][k];
+  }
   __syncthreads();
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t p = (uint64_t)blockIdx.x * (kXW / 4) + (wave >> 2);
-  const uint32_t k = wave & 3;
-  const bool active = p < batch;  // every wave reaches the barrier below
-  c64 *x = smem + kXTw + wave * S::lds_words;
-  const uint32_t l = threadIdx.x & 63u;
-  const c64 *wk = F + (active ? p : 0) * 4ull * n + k * n;  // weight spectrum W_k
+  // inverse of W_k, k = li: 2^(16 k) D_k, rounded
+  const uint32_t k = li;
   const double inv_n = 1.0 / (double)n;
   double worst = 0.0;
-  c64 v[16];
-#pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) v[m] = ld_stream(&wk[l + S::T * m]);
-  r16w_transform<false, 1>(v, v, x, l, ltw);
   uint64_t part[32];
-  const c64 *tws = launder(twist);
+  {
+    c64 v[16];
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) {
-    const uint32_t j = l + S::T * S::end_m(s);
-    const c64 tz = tws[j];
-    const c64 r = cmul(mk(tz.re * inv_n, (-tz.im) * inv_n), v[s]);
-    double d0, d1;
-    const uint64_t x0 = nearest_u64(r.re, d0), x1 = nearest_u64(r.im, d1);
-    worst = fmax(worst, fmax(d0, d1));
-    part[2 * s] = x0 << (16 * k);
-    part[2 * s + 1] = x1 << (16 * k);
+    for (uint32_t m = 0; m < 16; ++m) v[m] = x[l + S::T * m];
+    r16w_transform<false, 1>(v, v, x, l, ltw);
+    const c64 *tws = launder(twist);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const c64 tz = tws[j];
+      const c64 r = cmul(mk(tz.re * inv_n, (-tz.im) * inv_n), v[s]);
+      double d0, d1;
+      const uint64_t x0 = nearest_u64(r.re, d0), x1 = nearest_u64(r.im, d1);
+      worst = fmax(worst, fmax(d0, d1));
+      part[2 * s] = x0 << (16 * k);
+      part[2 * s + 1] = x1 << (16 * k);
+    }
   }
   // the transform's last LDS reads are behind its final wave fence; reuse the
   // buffer for this wave's 32 words per lane
   uint64_t *xb = reinterpret_cast<uint64_t *>(x);
 #pragma unroll
   for (uint32_t q = 0; q < 32; ++q) xb[q * 64 + l] = part[q];
+  for (uint32_t off = 1; off < 64; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
+  if (l == 0) s_worst[wave] = worst;
   __syncthreads();
   if (active && k == 0) {
     const uint64_t *b1 = reinterpret_cast<const uint64_t *>(x + 1 * S::lds_words);
@@ -279,12 +272,9 @@ k_exact_inv_r16(const c64 *__restrict__ F, uint64_t *__restrict__ C, double *mar
       st_stream(part[2 * s] + b1[q0] + b2[q0] + b3[q0], &c[j]);
       st_stream(part[2 * s + 1] + b1[q1] + b2[q1] + b3[q1], &c[j + n]);
     }
-  }
-  if (margin) {
-    for (uint32_t off = 1; off < 64; off <<= 1) worst = fmax(worst, __shfl_xor(worst, (int)off));
-    if (active && l == 0)
-      atomicMax(reinterpret_cast<unsigned long long *>(margin + p),
-                (unsigned long long)__double_as_longlong(worst));
+    if (margin && l == 0)
+      margin[p] = fmax(fmax(s_worst[wave], s_worst[wave + 1]),
+                       fmax(s_worst[wave + 2], s_worst[wave + 3]));
   }
 }
 
@@ -529,26 +519,20 @@ hipError_t exact_polymul(const Program &fwd, const Program &inv, const uint64_t 
 }
 
 // Register-resident form: Dif16 plan with n = 1024; tw_r16 is the plan's
-// lane-major twiddle block.
+// lane-major twiddle block.  One fused kernel, two products per workgroup,
+// no scratch.
 hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
-                             uint64_t batch, const dc64 *tw_r16, const dc64 *twist, dc64 *F,
+                             uint64_t batch, const dc64 *tw_r16, const dc64 *twist,
                              hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_fwd_r16), kXLds);
-  if (e == hipSuccess)
-    e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_inv_r16), kXLds);
+  constexpr size_t lds = kXLds + kXW * sizeof(double);  // + each wave's margin
+  hipError_t e = cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_exact_r16), lds);
   if (e != hipSuccess) return e;
   const c64 *t = reinterpret_cast<const c64 *>(tw_r16);
   const c64 *z = reinterpret_cast<const c64 *>(twist);
-  c64 *f = reinterpret_cast<c64 *>(F);
-  const uint64_t units = batch * 4;
-  hipLaunchKernelGGL(k_exact_fwd_r16, dim3((uint32_t)((units + kXW - 1) / kXW)), dim3(64 * kXW),
-                     kXLds, s, a, b, f, margin, batch, t, z);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
   const uint64_t per_wg = kXW / 4;  // products per workgroup (4 waves each)
-  hipLaunchKernelGGL(k_exact_inv_r16, dim3((uint32_t)((batch + per_wg - 1) / per_wg)),
-                     dim3(64 * kXW), kXLds, s, f, c, margin, batch, t, z);
+  hipLaunchKernelGGL(k_exact_r16, dim3((uint32_t)((batch + per_wg - 1) / per_wg)), dim3(64 * kXW),
+                     lds, s, a, b, c, margin, batch, t, z);
   return hipGetLastError();
 }
 

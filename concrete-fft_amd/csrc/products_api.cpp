@@ -22,7 +22,7 @@ hipError_t exact_polymul_w2(const uint64_t *a, const uint64_t *b, uint64_t *c, d
                             uint64_t batch, const dc64 *tw, const dc64 *tw_blk,
                             const dc64 *twist, const dc64 *untwist, dc64 *F, hipStream_t s);
 hipError_t exact_polymul_r16(const uint64_t *a, const uint64_t *b, uint64_t *c, double *margin,
-                             uint64_t batch, const dc64 *tw_r16, const dc64 *twist, dc64 *F,
+                             uint64_t batch, const dc64 *tw_r16, const dc64 *twist,
                              hipStream_t s);
 hipError_t torus_to_fourier(const cfft_prog::Program &fwd, const uint64_t *key, dc64 *out,
                             uint64_t npolys, const dc64 *tw, const dc64 *twist, hipStream_t s);
@@ -180,19 +180,20 @@ int cfft_exact_polymul_u64(cfft_plan plan, const uint64_t *a, const uint64_t *b,
   if (plan->n < 2 || plan->n > 2048 || plan->fwd_full.npass == 0)
     return set_error(CFFT_ERROR_SIZE, "exact product supports 4 <= N <= 4096 (fft_size <= 2048)");
   hipStream_t s = (hipStream_t)stream;
-  // the wave-local forms keep the 4 weight spectra of a product, the generic
+  // Dif16 with base_n == n == 1024: one fused kernel, no scratch
+  if (plan->n == 1024 && plan->d_tw_r16)
+    return hip_error(cfft_launch::exact_polymul_r16(a, b, c, margin, batch, plan->d_tw_r16,
+                                                    plan->d_twist, s),
+                     "exact product");
+  // the wave-local form keeps the 4 weight spectra of a product, the generic
   // form its 8 limb spectra
-  const bool r16 = plan->n == 1024 && plan->d_tw_r16;           // Dif16, base_n == n
-  const bool w2 = !r16 && plan->n == 2048 && plan->d_tw_base1024;  // Dif16, base_n == 1024
-  const size_t bytes = batch * ((r16 || w2) ? 4 : 8) * plan->n * sizeof(cfft_launch::dc64);
+  const bool w2 = plan->n == 2048 && plan->d_tw_base1024;  // Dif16, base_n == 1024
+  const size_t bytes = batch * (w2 ? 4 : 8) * plan->n * sizeof(cfft_launch::dc64);
   cfft_internal::ScratchLease lease;
   hipError_t e = cfft_internal::scratch_acquire(bytes, s, &lease);
   if (e != hipSuccess) return hip_error(e, "exact product scratch");
   void *F = lease.ptr;
-  if (r16)
-    e = cfft_launch::exact_polymul_r16(a, b, c, margin, batch, plan->d_tw_r16, plan->d_twist,
-                                       reinterpret_cast<cfft_launch::dc64 *>(F), s);
-  else if (w2)
+  if (w2)
     e = cfft_launch::exact_polymul_w2(a, b, c, margin, batch, plan->d_tw, plan->d_tw_base1024,
                                       plan->d_twist, plan->d_untwist,
                                       reinterpret_cast<cfft_launch::dc64 *>(F), s);
