@@ -304,31 +304,25 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   };
   // Fourier multiply-accumulate row by row: each row's key and spectrum
   // points of the lane are loaded together (one memory round trip per row,
-  // not per point); every point still sums its rows in ascending order.  At
-  // n = 2048 the second half accumulates beside the first, so its rows are
-  // loaded 8 points at a time (16 would spill).
-  constexpr uint32_t MC = LOG2N > 10 ? 8 : 16;
+  // not per point); every point still sums its rows in ascending order
   auto mac_rows = [&](c64 *acc, uint32_t base) {
 #pragma unroll
     for (uint32_t m = 0; m < 16; ++m) acc[m] = mk(0.0, 0.0);
     for (uint32_t r = 0; r < rows; ++r) {
       const c64 *kr = keyf + ((uint64_t)r * k1 + c) * n + base + l;
       const c64 *sr = sp + (uint64_t)r * n + base + l;
+      c64 kv[16], sv[16];
 #pragma unroll
-      for (uint32_t m0 = 0; m0 < 16; m0 += MC) {
-        c64 kv[MC], sv[MC];
-#pragma unroll
-        for (uint32_t m = 0; m < MC; ++m) {
-          kv[m] = kr[T * (m0 + m)];
-          sv[m] = sr[T * (m0 + m)];
-        }
-#pragma unroll
-        for (uint32_t m = 0; m < MC; ++m) {
-          const c64 pr = cmul(kv[m], sv[m]);
-          acc[m0 + m] = mk(acc[m0 + m].re + pr.re, acc[m0 + m].im + pr.im);
-        }
-        sched_fence();  // one row's (or half row's) loads in flight at a time
+      for (uint32_t m = 0; m < 16; ++m) {
+        kv[m] = kr[T * m];
+        sv[m] = sr[T * m];
       }
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const c64 pr = cmul(kv[m], sv[m]);
+        acc[m] = mk(acc[m].re + pr.re, acc[m].im + pr.im);
+      }
+      sched_fence();  // one row's loads in flight at a time
     }
   };
   if constexpr (LOG2N <= 10) {
@@ -341,7 +335,16 @@ k_extprod_cols_w(const c64 *__restrict__ keyf, const c64 *__restrict__ S,
   } else {
     c64 va[16], vb[16];
     mac_rows(va, 0);
+    // the first half waits in the wave's exchange buffer while the second
+    // accumulates (both halves' sums beside a row's loads would spill)
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) x[m * 64 + l] = va[m];
     mac_rows(vb, 1024);
+    wave_lds_fence();
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) va[m] = x[m * 64 + l];
+    wave_lds_fence();  // read back before the transform reuses the buffer
     r16l_2048_transform<false>(va, vb, x, l, ltw, tw);
 #pragma unroll
     for (uint32_t s = 0; s < 16; ++s) {
