@@ -174,15 +174,24 @@ hipError_t set_lds(const void *kernel, size_t bytes) {
 
 // workgroup geometry of the LDS kernels: E points per thread, about 34 KiB of
 // LDS per workgroup (several workgroups per CU) and at most 512 threads.
+// E = 16 only for programs with a radix-16 pass: the 16-point form of the
+// executor needs about twice the registers of the 8-point one.
 struct Geom {
   int E;
   uint32_t T, G, threads;
   size_t lds;
 };
-Geom geom_for(uint32_t log2n, uint32_t per_wg_points) {
+uint32_t max_radix(const Program &prog) {
+  uint32_t r = 2;
+  for (uint32_t i = 0; i < prog.npass; ++i) r = prog.p[i].radix > r ? prog.p[i].radix : r;
+  return r;
+}
+Geom geom_for(uint32_t log2n, uint32_t per_wg_points, uint32_t radix) {
   Geom g;
   const uint32_t n = 1u << log2n;
-  g.E = n >= 16 ? 16 : (int)n;
+  // (8 points per thread up to 4096 points: at most 512 threads per transform)
+  const uint32_t e_max = radix > 8 || n > 4096 ? 16u : 8u;
+  g.E = (int)(n >= e_max ? e_max : n);
   g.T = n / (uint32_t)g.E;
   g.G = per_wg_points > n ? per_wg_points / n : 1;
   if (g.T * g.G > 512) g.G = 512 / g.T > 0 ? 512 / g.T : 1;
@@ -198,7 +207,7 @@ namespace cfft_launch {
 hipError_t fft_lds(const Program &prog, dc64 *data, uint64_t ntrans, uint32_t chunks_log2,
                    uint64_t stride, const dc64 *tw, hipStream_t s) {
   if (ntrans == 0 || prog.npass == 0) return hipSuccess;
-  Geom g = geom_for(prog.log2n, 2048);
+  Geom g = geom_for(prog.log2n, 2048, max_radix(prog));
   const uint64_t blocks = (ntrans + g.G - 1) / g.G;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw);
@@ -324,7 +333,8 @@ hipError_t polymul_generic(const Program &fwd, const Program &inv, const uint64_
                            const int64_t *b, uint64_t *c, uint64_t batch, const dc64 *tw,
                            const dc64 *twist, const dc64 *untwist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  Geom g = geom_for(fwd.log2n, 2048);
+  const uint32_t rf = max_radix(fwd), ri = max_radix(inv);
+  Geom g = geom_for(fwd.log2n, 2048, rf > ri ? rf : ri);
   const uint64_t blocks = (batch + g.G - 1) / g.G;
   const c64 *t = reinterpret_cast<const c64 *>(tw);
   const c64 *tws = reinterpret_cast<const c64 *>(twist);
