@@ -60,6 +60,7 @@ cfft_plan_s::~cfft_plan_s() {
   if (d_pos) (void)hipFree(d_pos);
   if (d_twist) (void)hipFree(d_twist);
   if (d_tw_r16) (void)hipFree(d_tw_r16);
+  if (d_tw_dit16) (void)hipFree(d_tw_dit16);
   if (d_tw_base1024) (void)hipFree(d_tw_base1024);
   if (d_tw_r8) (void)hipFree(d_tw_r8);
   if (d_tw_chunk) (void)hipFree(d_tw_chunk);
@@ -201,15 +202,17 @@ int build_f64_plan(cfft_plan_s *p) {
   std::vector<cfft_host::C64> tw = cfft_host::twiddle_table(n);
   CFFT_HIP(hipMalloc(&p->d_tw, n * sizeof(dc64)));
   CFFT_HIP(hipMemcpy(p->d_tw, tw.data(), n * sizeof(dc64), hipMemcpyHostToDevice));
-  if (p->algo == CFFT_DIF16 && p->base_n == n && cfft_launch::r16_supported(log2n)) {
+  const bool dit16 = p->algo == CFFT_DIT16 && p->base_n == n && n == 1024;
+  if ((p->algo == CFFT_DIF16 && p->base_n == n && cfft_launch::r16_supported(log2n)) || dit16) {
     const size_t T = n / 16, T2 = T / 16;
     std::vector<cfft_host::C64> blk(16 * T + 16 * T2);
     for (size_t k = 0; k < 16; ++k) {
       for (size_t l = 0; l < T; ++l) blk[k * T + l] = tw[(k * l) % n];
       for (size_t q = 0; q < T2; ++q) blk[16 * T + k * T2 + q] = tw[(16 * k * q) % n];
     }
-    CFFT_HIP(hipMalloc(&p->d_tw_r16, blk.size() * sizeof(dc64)));
-    CFFT_HIP(hipMemcpy(p->d_tw_r16, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
+    dc64 **dst = dit16 ? &p->d_tw_dit16 : &p->d_tw_r16;
+    CFFT_HIP(hipMalloc(dst, blk.size() * sizeof(dc64)));
+    CFFT_HIP(hipMemcpy(*dst, blk.data(), blk.size() * sizeof(dc64), hipMemcpyHostToDevice));
   }
   if (p->algo == CFFT_DIF16 && p->base_n == n && (n == 256 || n == 512)) {
     // the wave-local n = 256 / 512 kernels read the 1024-point block at
@@ -346,6 +349,9 @@ int run_transform(const cfft_plan_s *p, void *buf, size_t batch, size_t stride, 
   if (p->d_tw_r8)
     return hip_error(cfft_launch::r8_transform(fwd, d, batch, stride, p->d_tw_r8, s),
                      "r8_transform");
+  if (p->d_tw_dit16)  // Dit16, n = base_n = 1024
+    return hip_error(cfft_launch::dit16_transform(fwd, d, batch, stride, p->d_tw_dit16, s),
+                     "dit16_transform");
   if (p->n == 1024 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, base_n = 512
     return hip_error(cfft_launch::wd2_512_transform(fwd, d, batch, stride, p->d_tw,
                                                     p->d_tw_base1024, s),
@@ -959,6 +965,9 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
   if (p->d_tw_r8)
     return hip_error(cfft_launch::r8_polymul(a, b, c, batch, p->d_tw_r8, p->d_twist, s),
                      "r8_polymul");
+  if (p->d_tw_dit16)
+    return hip_error(cfft_launch::dit16_polymul(a, b, c, batch, p->d_tw_dit16, p->d_twist, s),
+                     "dit16_polymul");
   return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
                                                 p->d_twist, p->d_untwist, s),
                    "polymul");

@@ -107,6 +107,14 @@ hipError_t depth16k_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t str
 hipError_t depth8k_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                              const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
 
+// Register-resident kernels for Dit16 plans with base_n == n == 1024
+// (r16dit_exec.hpp): transforms (in place, lane-linear order) and the fused
+// negacyclic product at N = 2048; tw_r16 is the plan's r16 twiddle block.
+hipError_t dit16_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
+                           const dc64 *tw_r16, hipStream_t s);
+hipError_t dit16_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                         const dc64 *tw_r16, const dc64 *twist, hipStream_t s);
+
 // Register-resident kernels for Dif8 plans with base_n == n == 1024 (r8_exec.hpp).
 bool r8_supported(uint32_t log2n);
 hipError_t r8_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride, const dc64 *tw,

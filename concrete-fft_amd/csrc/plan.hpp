@@ -33,6 +33,9 @@ struct cfft_plan_s {
   // r16 (register-resident Dif16) twiddle block: T[k*l] as [k][l] (16 x n/16)
   // then T[16 k p] as [k][p] (16 x n/256)
   cfft_launch::dc64 *d_tw_r16 = nullptr;
+  // Dit16 plans with n = base_n = 1024: the same block, for the
+  // register-resident Dit16 kernels
+  cfft_launch::dc64 *d_tw_dit16 = nullptr;
   // Dif16 plans with base_n = 1024 < n <= 8192, and with n = base_n = 512:
   // the r16 twiddle block of a 1024-point transform (T[k l n/1024] and
   // T[16 k p n/1024], the same values as a 1024-point plan's d_tw_r16; at

@@ -274,3 +274,30 @@ def test_measure_method_picks_a_valid_plan(cfft):
     p.fwd(z)
     p.inv(z)
     assert np.max(np.abs(z / 2048 - x)) < 1e-12
+
+
+@pytest.mark.parametrize("batch,stride", [(1, 1024), (9, 1024), (5, 1040)])
+def test_dit16_register_form(cfft, oracle, batch, stride):
+    """Dit16 with n = base_n = 1024 runs the register-resident Dit16 kernel
+    (one transform per wave): bit-exact against the oracle, strided batches
+    leave the padding alone."""
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    import cfft_amd
+    n = 1024
+    rng = np.random.default_rng(batch + stride)
+    x = rand_c(rng, (batch, stride))
+    plan = Plan(n, Method.UserProvided(FftAlgo.Dit16, 1024))
+    s = Stream()
+    d = DeviceBuffer.from_host(x)
+    cfft_amd.check(cfft_amd.lib.cfft_fwd(plan.handle, d.dptr, batch, stride, s.handle))
+    s.synchronize()
+    y = d.download()
+    ref = oracle.unordered_fwd(np.ascontiguousarray(x[:, :n]), 7, 1024)
+    assert np.array_equal(y[:, :n], ref)
+    assert np.array_equal(y[:, n:], x[:, n:])
+    cfft_amd.check(cfft_amd.lib.cfft_inv(plan.handle, d.dptr, batch, stride, s.handle))
+    s.synchronize()
+    w = d.download()
+    assert np.array_equal(w[:, :n], oracle.unordered_inv(ref, 7, 1024))
+    assert np.array_equal(w[:, n:], x[:, n:])

@@ -21,7 +21,8 @@ def torus_err(c, exact):
 @pytest.mark.parametrize("N,algo,base_n", [(2048, 6, 1024), (2048, 2, 32), (2048, 7, 512),
                                            (1024, 4, 512), (4096, 3, 1024), (256, 0, 128),
                                            (64, 6, 32), (16384, 6, 1024), (2048, 4, 1024),
-                                           (4096, 6, 1024), (8192, 6, 1024), (512, 6, 256)])
+                                           (4096, 6, 1024), (8192, 6, 1024), (512, 6, 256),
+                                           (2048, 7, 1024), (2048, 5, 1024)])
 def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
