@@ -968,6 +968,10 @@ int cfft_polymul_dispatch(const cfft_plan_s *p, const uint64_t *a, const int64_t
   if (p->d_tw_dit16)
     return hip_error(cfft_launch::dit16_polymul(a, b, c, batch, p->d_tw_dit16, p->d_twist, s),
                      "dit16_polymul");
+  if (p->n == 1024 && p->base_n == 512 && p->d_tw_base1024)  // Dif16, base_n = 512
+    return hip_error(cfft_launch::wd2_512_polymul(a, b, c, batch, p->d_tw, p->d_tw_base1024,
+                                                  p->d_twist, s),
+                     "wd2_512_polymul");
   return hip_error(cfft_launch::polymul_generic(p->fwd_full, p->inv_full, a, b, c, batch, p->d_tw,
                                                 p->d_twist, p->d_untwist, s),
                    "polymul");

@@ -72,6 +72,11 @@ hipError_t wsub_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uin
 // 1024-point r16 block.
 hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stride,
                              const dc64 *tw, const dc64 *tw_blk, hipStream_t s);
+// the fused negacyclic product (N = 2048) on the same plans: one product per
+// wavefront, the depth pass's partner values swapped between its half-waves
+hipError_t wd2_512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                           const dc64 *tw, const dc64 *tw_blk, const dc64 *twist,
+                           hipStream_t s);
 
 // Dif16 plans with n = base_n = 1024: two transforms per wavefront
 // (r16w_transform), 8 wavefronts per workgroup; tw_r16 is the plan's r16

@@ -601,6 +601,168 @@ k_wd2_512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
   }
 }
 
+// ---- negacyclic product, N = 2048 on a Dif16 plan with base_n = 512 ----
+// Persistent: one product per wavefront at a time, 8 waves per workgroup, at
+// most one workgroup per CU, products strided over the waves; B of a wave's
+// next product is loaded during its inverse.  The plan's radix-2 depth pass
+// (src/unordered.rs:391-489) pairs the 1024-point transform's points j and
+// 512 + j.  Lane 32 h + l (h = 0, 1; l < 32) holds chunk h -- points
+// 512 h + l + 32 m in its slots m -- so each depth butterfly takes one value
+// from each half-wave: the two halves swap their values through the wave's
+// exchange buffer (slot m of lane i at 64 m + i), and each then forms its own
+// output of the butterfly (a + b in chunk 0; in chunk 1 a - b, written as
+// b' + (-a') with a' its own value, which is the same IEEE operation).  Each
+// half then runs its chunk's 512-point base transforms (r16l_transform<9>, A
+// and B together, in its own 544-point half of the buffer).  Otherwise the
+// operations of the generic fused product: fold + twist at the loads,
+// pointwise product, inverse (base transforms, then the depth pass with
+// conj(T[q]) on chunk 1's point before the butterfly), untwist with the
+// exact 1/n and torus rounding at the stores.
+
+// the depth butterfly's partner swap for slots m0 .. m0 + 7 of one operand
+__device__ __forceinline__ void half_swap_write(const c64 *u, c64 *xw, uint32_t lane,
+                                                uint32_t m0, uint32_t base) {
+#pragma unroll
+  for (uint32_t m = 0; m < 8; ++m) xw[base + 64 * m + lane] = u[m0 + m];
+}
+
+__global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_wd2_512_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
+                  uint64_t *__restrict__ C, uint64_t batch, const c64 *__restrict__ tw,
+                  const c64 *__restrict__ tw_blk, const c64 *__restrict__ twist) {
+  using S = R16<9>;
+  constexpr uint32_t n = 1024;
+  static_assert(2 * S::lds_words == R16<10>::lds_words, "two half-wave buffers per wave");
+  extern __shared__ c64 smem[];
+  c64 *ltw = smem;
+  for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  c64 *xw = smem + kTwWords + wave * R16<10>::lds_words;  // the wave's 1088 points
+  const uint64_t step = (uint64_t)gridDim.x * kPW;
+  uint64_t p = (uint64_t)blockIdx.x * kPW + wave;
+  uint64_t pb[32];  // raw words of B for the wave's current product
+  if (p < batch) {
+    const uint32_t j0 = 512 * ((threadIdx.x >> 5) & 1u) + (threadIdx.x & 31u);
+    const int64_t *b = B + p * (2 * n);
+#pragma unroll
+    for (uint32_t m = 0; m < 16; ++m) {
+      pb[2 * m] = (uint64_t)ld_stream(&b[j0 + 32 * m]);
+      pb[2 * m + 1] = (uint64_t)ld_stream(&b[j0 + 32 * m + n]);
+    }
+  }
+  for (; p < batch; p += step) {
+    // recomputed per product so per-lane addresses are not kept live across
+    // the loop
+    const uint32_t lane = launder_u32(threadIdx.x) & 63u, h = lane >> 5, l = lane & 31u;
+    const uint32_t pl = lane ^ 32u;  // the other half's lane with the same l
+    const uint64_t *a = A + p * (2 * n);
+    uint64_t *c = C + p * (2 * n);
+    c64 *x = xw + h * S::lds_words;  // this half's 544 points
+    const double inv_n = 1.0 / (double)n;
+    c64 fa[16], v[16];
+    {
+      uint64_t ra[32];
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = 512 * h + l + 32 * m;
+        ra[2 * m] = ld_stream(&a[j]);
+        ra[2 * m + 1] = ld_stream(&a[j + n]);
+      }
+      sched_fence();
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const c64 z = twist[512 * h + l + 32 * m];
+        fa[m] = cmul(z, mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
+        v[m] = cmul(z, mk(i64_to_f64((int64_t)pb[2 * m]), i64_to_f64((int64_t)pb[2 * m + 1])));
+      }
+    }
+    // DEPTH_FWD radix 2: chunk 0 gets a + b, chunk 1 (a - b) T[j], j = l + 32 m;
+    // eight slots of A and B per swap (1024 points of the buffer)
+    {
+      const c64 *twd = launder(tw);
+#pragma unroll
+      for (uint32_t m0 = 0; m0 < 16; m0 += 8) {
+        wave_lds_fence();
+        half_swap_write(fa, xw, lane, m0, 0);
+        half_swap_write(v, xw, lane, m0, 512);
+        wave_lds_fence();
+#pragma unroll
+        for (uint32_t m1 = 0; m1 < 8; m1 += 4) {
+          c64 w[4];
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) w[e] = twd[l + 32 * (m0 + m1 + e)];
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) {
+            const uint32_t m = m0 + m1 + e;
+            const c64 oa = xw[64 * (m1 + e) + pl], ob = xw[512 + 64 * (m1 + e) + pl];
+            const c64 sa = h ? mk(-fa[m].re, -fa[m].im) : fa[m];
+            const c64 sb = h ? mk(-v[m].re, -v[m].im) : v[m];
+            const c64 ta = cadd(oa, sa), tb = cadd(ob, sb);
+            fa[m] = h ? cmul(w[e], ta) : ta;
+            v[m] = h ? cmul(w[e], tb) : tb;
+          }
+          sched_fence();
+        }
+      }
+      wave_lds_fence();
+    }
+    r16l_transform<9, true, 2>(fa, v, x, l, ltw);
+    c64 z[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
+    r16_end_to_input<9>(fa, z);
+    sched_fence();
+    {
+      // B of the wave's next product (past its last product: the L2-resident
+      // twist table, 2048 words, never used)
+      const uint64_t nx = p + step;
+      const int64_t *bn = nx < batch ? B + nx * (2 * n) : reinterpret_cast<const int64_t *>(twist);
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = 512 * h + l + 32 * m;
+        pb[2 * m] = (uint64_t)ld_stream(&bn[j]);
+        pb[2 * m + 1] = (uint64_t)ld_stream(&bn[j + n]);
+      }
+    }
+    sched_fence();
+    r16l_transform<9, false, 1>(z, z, x, l, ltw);
+    // DEPTH_INV radix 2: u1 = conj(T[q]) chunk 1's point; chunk 0 gets u0 + u1,
+    // chunk 1 u0 - u1, q = l + 32 end_m(s)
+    {
+      const c64 *twd = launder(tw);
+#pragma unroll
+      for (uint32_t s0 = 0; s0 < 16; s0 += 4) {
+        c64 w[4];
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+          w[e] = twd[l + 32 * S::end_m(s0 + e)];
+          w[e].im = -w[e].im;
+        }
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) z[s0 + e] = h ? cmul(w[e], z[s0 + e]) : z[s0 + e];
+      }
+      wave_lds_fence();
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) xw[64 * s + lane] = z[s];
+      wave_lds_fence();
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) {
+        const c64 o = xw[64 * s + pl];
+        z[s] = cadd(o, h ? mk(-z[s].re, -z[s].im) : z[s]);
+      }
+    }
+    const c64 *tws = launder(twist);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = 512 * h + l + 32 * S::end_m(s);
+      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
+      st_stream(from_torus(r.re), &c[j]);
+      st_stream(from_torus(r.im), &c[j + n]);
+    }
+  }
+}
+
 }  // namespace
 
 namespace cfft_launch {
@@ -623,6 +785,24 @@ hipError_t wd2_512_transform(bool fwd, dc64 *data, uint64_t batch, uint64_t stri
   else
     hipLaunchKernelGGL(k_wd2_512_transform<false>, grid, dim3(64 * kPW), kWaveLds, s, d, batch,
                        stride, t, tb);
+  return hipGetLastError();
+}
+
+hipError_t wd2_512_polymul(const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                           const dc64 *tw, const dc64 *tw_blk, const dc64 *twist,
+                           hipStream_t s) {
+  if (batch == 0) return hipSuccess;
+  hipError_t e =
+      cfft_launch::ensure_lds(reinterpret_cast<const void *>(k_wd2_512_polymul), kWaveLds);
+  if (e != hipSuccess) return e;
+  int cus = 0;
+  e = cu_count(&cus);
+  if (e != hipSuccess) return e;
+  const uint64_t want = (batch + kPW - 1) / kPW;
+  const dim3 grid((uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus));
+  hipLaunchKernelGGL(k_wd2_512_polymul, grid, dim3(64 * kPW), kWaveLds, s, a, b, c, batch,
+                     reinterpret_cast<const c64 *>(tw), reinterpret_cast<const c64 *>(tw_blk),
+                     reinterpret_cast<const c64 *>(twist));
   return hipGetLastError();
 }
 

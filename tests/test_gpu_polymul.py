@@ -22,7 +22,7 @@ def torus_err(c, exact):
                                            (1024, 4, 512), (4096, 3, 1024), (256, 0, 128),
                                            (64, 6, 32), (16384, 6, 1024), (2048, 4, 1024),
                                            (4096, 6, 1024), (8192, 6, 1024), (512, 6, 256),
-                                           (2048, 7, 1024), (2048, 5, 1024)])
+                                           (2048, 7, 1024), (2048, 5, 1024), (2048, 6, 512)])
 def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
@@ -143,6 +143,23 @@ def test_persistent_product_batches(cfft, oracle, batch):
     plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 1024))
     c = products.negacyclic_polymul_u64_host(plan, a, b)
     ref = oracle.polymul_f64(a, b, 6, 1024, threads=16)
+    assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
+
+
+@pytest.mark.parametrize("batch", [1, 7, 9, 1031, 2049, 4100])
+def test_base512_product_batches(cfft, oracle, batch):
+    """N = 2048 on the Dif16 / 512 plan (the persistent kernel: one product
+    per wave at a time, at most one workgroup per CU, the depth pass's
+    partner values swapped between the half-waves, the next product's B
+    loaded during the inverse): bit-exact against the oracle with partial
+    workgroups and for batches past a full grid of 256 workgroups."""
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    a, b = rand_inputs(500 + batch, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 512))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, 6, 512, threads=16)
     assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
 
 
