@@ -108,6 +108,10 @@ struct Bufs {
 };
 
 __host__ __device__ __forceinline__ uint32_t pad128(uint32_t i) { return i + (i >> 5); }
+// blocks of 2^L points per k_fft128 workgroup: enough to fill one wave
+__host__ __device__ __forceinline__ uint32_t fft128_blocks_per_wg(uint32_t L) {
+  return L >= 9 ? 1u : 1u << (9 - L);
+}
 
 // Position of point `u` (3-bit combination, most significant = first stage
 // of the round) of group `g`, for a round whose top group bit is `ptop` and
@@ -241,10 +245,12 @@ __device__ __forceinline__ void run_round(double *lds, const Bufs &gbuf, bool fi
   __syncthreads();
 }
 
-// One workgroup per block of n = 2^L points (L >= 3), T = n / 8 threads:
-// the rounds over bits 0 .. L-1 of a transform of 2^Lg points (Lg >= L)
-// whose blocks are contiguous; workgroup b holds block b % 2^(Lg-L) of
-// transform b / 2^(Lg-L), at b * n.
+// Blocks of n = 2^L points (L >= 3), T = n / 8 threads each: the rounds over
+// bits 0 .. L-1 of a transform of 2^Lg points (Lg >= L) whose blocks are
+// contiguous; block b is block b % 2^(Lg-L) of transform b / 2^(Lg-L), at
+// b * n.  A workgroup holds G = max(1, 64 / T) blocks (at least one full
+// wave), each with its own LDS slice; a block past the last is idle but
+// takes part in every barrier.
 template <bool FWD, int MULV>
 __global__ void __launch_bounds__(512) k_fft128(double *re0, double *re1, double *im0,
                                                 double *im1, const double *__restrict__ t_re0,
@@ -252,12 +258,14 @@ __global__ void __launch_bounds__(512) k_fft128(double *re0, double *re1, double
                                                 const double *__restrict__ t_im0,
                                                 const double *__restrict__ t_im1, uint32_t L,
                                                 uint32_t Lg, uint64_t blocks) {
-  extern __shared__ double lds[];
+  extern __shared__ double smem[];
   const uint32_t n = 1u << L;
   const uint32_t T = n / 8;
-  const uint64_t tr = blockIdx.x;
+  const uint32_t G = fft128_blocks_per_wg(L);
+  const uint32_t sub = threadIdx.x >> (L - 3), lt = threadIdx.x & (T - 1);
+  const uint64_t tr = (uint64_t)blockIdx.x * G + sub;
   const bool active = tr < blocks;
-  const uint32_t lt = threadIdx.x;
+  double *lds = smem + (size_t)sub * 4 * pad128(n);
   const uint32_t jhi = (uint32_t)(tr & ((1ull << (Lg - L)) - 1)) << L;
   Bufs g{re0 + tr * n, re1 + tr * n, im0 + tr * n, im1 + tr * n};
   Tw tw{t_re0, t_re1, t_im0, t_im1};
@@ -513,12 +521,14 @@ hipError_t fft128_blocks(uint32_t L, uint32_t Lg, double *re0, double *re1, doub
                          const double *t_im0, const double *t_im1, uint64_t batch, hipStream_t s) {
   const uint32_t n = 1u << L;
   const uint64_t blocks = batch << (Lg - L);
-  const size_t lds = 4 * (size_t)pad128(n) * sizeof(double);
+  const uint32_t G = fft128_blocks_per_wg(L);
+  const size_t lds = (size_t)G * 4 * pad128(n) * sizeof(double);
   const void *k = reinterpret_cast<const void *>(k_fft128<FWD, MULV>);
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_fft128<FWD, MULV>), dim3((uint32_t)blocks), dim3(n / 8), lds, s, re0, re1,
-                     im0, im1, t_re0, t_re1, t_im0, t_im1, L, Lg, blocks);
+  const uint64_t wgs = (blocks + G - 1) / G;
+  hipLaunchKernelGGL((k_fft128<FWD, MULV>), dim3((uint32_t)wgs), dim3(G * (n / 8)), lds, s, re0,
+                     re1, im0, im1, t_re0, t_re1, t_im0, t_im1, L, Lg, blocks);
   return hipGetLastError();
 }
 

@@ -179,6 +179,33 @@ def test_gpu_free_functions_bit_exact(cfft, O, n, mulv):
 
 
 @pytest.mark.gpu
+# below 512 points a workgroup holds 64 / (n / 8) transforms (one wave); 37
+# transforms leave the last workgroup partly idle
+@pytest.mark.parametrize("n", [8, 16, 64, 128, 256])
+def test_gpu_free_functions_packed_batches(cfft, O, n):
+    import cfft_amd
+    print("fft128 packed batch: n=%d" % n, flush=True)
+    rng = np.random.default_rng(100 + n)
+    batch = 37
+    tw = O.fft128_twiddles(n)
+    x = [rng.uniform(-1, 1, (batch, n)) for _ in range(4)]
+    x[1] *= 2.0 ** -54
+    x[3] *= 2.0 ** -54
+    d = dev(x)
+    dtw = dev(tw)
+    cfft_amd.fft128.negacyclic_fwd_fft(*d, dtw, batch=batch, mul_variant=1)
+    got = [b.download() for b in d]
+    ref = O.fft128_fwd([a.reshape(-1) for a in x], tw, 1)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g.reshape(-1), r), n
+    cfft_amd.fft128.negacyclic_inv_fft(*d, dtw, batch=batch, mul_variant=1)
+    got2 = [b.download() for b in d]
+    ref2 = O.fft128_inv(ref, tw, 1)
+    for g, r in zip(got2, ref2):
+        assert np.array_equal(g.reshape(-1), r), n
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("N", [64, 256, 2048, 16384, 32768])
 def test_gpu_plan_product_bound(cfft, N):
     # test_wrapper (src/fft128/mod.rs:1972-2065) through fft128::Plan on the device
