@@ -508,8 +508,11 @@ k_w512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
 // kernel: fold + twist at the loads, both forward transforms together
 // (r16l_transform<LOG2N>, twiddle reads shared), pointwise product, inverse,
 // untwist and torus rounding at the stores.  8 waves per workgroup share the
-// 1024-point twiddle block in LDS (read at column stride 1024 / n).  A group
-// past the batch repeats the last product and stores nothing.
+// 1024-point twiddle block in LDS (read at column stride 1024 / n).
+// Persistent: at most one workgroup per CU, the waves step over the batch
+// 64 / T products at a time, and B of a group's next product is loaded
+// during the inverse.  A group past the batch repeats the last product and
+// stores nothing.
 template <int LOG2N>
 __global__ void __launch_bounds__(64 * kPW) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_wsub_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
@@ -524,39 +527,77 @@ k_wsub_polymul(const uint64_t *__restrict__ A, const int64_t *__restrict__ B,
   for (uint32_t i = threadIdx.x; i < kTwWords; i += blockDim.x) ltw[i] = tw_blk[i];
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * PW;
-  if (first >= batch) return;
-  const uint32_t grp = (threadIdx.x & 63u) / S::T, l = threadIdx.x % S::T;
-  const uint64_t p = first + grp;
-  const bool act = p < batch;
-  const uint64_t pc = act ? p : batch - 1;
-  const uint64_t *a = A + pc * (2 * n);
-  const int64_t *b = B + pc * (2 * n);
-  uint64_t *c = C + pc * (2 * n);
-  c64 *x = smem + kTwWords + wave * R16<10>::lds_words + grp * S::lds_words;
-  const double inv_n = 1.0 / (double)n;
-  c64 fa[16], v[16];
+  const uint64_t step = (uint64_t)gridDim.x * kPW * PW;
+  uint64_t first = ((uint64_t)blockIdx.x * kPW + wave) * PW;
+  uint64_t pb[32];  // raw words of B for the group's current product
+  if (first < batch) {
+    const uint32_t g0 = (threadIdx.x & 63u) / S::T, l0 = threadIdx.x % S::T;
+    const uint64_t p0 = first + g0 < batch ? first + g0 : batch - 1;
+    const int64_t *b = B + p0 * (2 * n);
 #pragma unroll
-  for (uint32_t m = 0; m < 16; ++m) {
-    const uint32_t j = l + S::T * m;
-    const c64 z = twist[j];
-    fa[m] = cmul(z, mk(torus_to_f64(ld_stream(&a[j])), torus_to_f64(ld_stream(&a[j + n]))));
-    v[m] = cmul(z, mk(i64_to_f64(ld_stream(&b[j])), i64_to_f64(ld_stream(&b[j + n]))));
+    for (uint32_t m = 0; m < 16; ++m) {
+      pb[2 * m] = (uint64_t)ld_stream(&b[l0 + S::T * m]);
+      pb[2 * m + 1] = (uint64_t)ld_stream(&b[l0 + S::T * m + n]);
+    }
   }
-  r16l_transform<LOG2N, true, 2>(fa, v, x, l, ltw);
-  c64 z[16];
+  for (; first < batch; first += step) {
+    // recomputed per product so per-lane addresses are not kept live across
+    // the loop
+    const uint32_t lane = launder_u32(threadIdx.x) & 63u;
+    const uint32_t grp = lane / S::T, l = lane % S::T;
+    const uint64_t p = first + grp;
+    const bool act = p < batch;
+    const uint64_t pc = act ? p : batch - 1;
+    const uint64_t *a = A + pc * (2 * n);
+    uint64_t *c = C + pc * (2 * n);
+    c64 *x = smem + kTwWords + wave * R16<10>::lds_words + grp * S::lds_words;
+    const double inv_n = 1.0 / (double)n;
+    c64 fa[16], v[16];
+    {
+      uint64_t ra[32];
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
-  r16_end_to_input<LOG2N>(fa, z);
-  r16l_transform<LOG2N, false, 1>(z, z, x, l, ltw);
-  const c64 *tws = launder(twist);
+      for (uint32_t m = 0; m < 16; ++m) {
+        const uint32_t j = l + S::T * m;
+        ra[2 * m] = ld_stream(&a[j]);
+        ra[2 * m + 1] = ld_stream(&a[j + n]);
+      }
+      sched_fence();
 #pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) {
-    const uint32_t j = l + S::T * S::end_m(s);
-    const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
-    if (act) {
-      st_stream(from_torus(r.re), &c[j]);
-      st_stream(from_torus(r.im), &c[j + n]);
+      for (uint32_t m = 0; m < 16; ++m) {
+        const c64 z = twist[l + S::T * m];
+        fa[m] = cmul(z, mk(torus_to_f64(ra[2 * m]), torus_to_f64(ra[2 * m + 1])));
+        v[m] = cmul(z, mk(i64_to_f64((int64_t)pb[2 * m]), i64_to_f64((int64_t)pb[2 * m + 1])));
+      }
+    }
+    r16l_transform<LOG2N, true, 2>(fa, v, x, l, ltw);
+    c64 z[16];
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) fa[s] = cmul(fa[s], v[s]);
+    r16_end_to_input<LOG2N>(fa, z);
+    sched_fence();
+    {
+      // B of the group's next product (past the batch: the L2-resident twist
+      // table, 2 n words, never used)
+      const uint64_t nx = p + step;
+      const int64_t *bn =
+          nx < batch ? B + nx * (2 * n) : reinterpret_cast<const int64_t *>(twist);
+#pragma unroll
+      for (uint32_t m = 0; m < 16; ++m) {
+        pb[2 * m] = (uint64_t)ld_stream(&bn[l + S::T * m]);
+        pb[2 * m + 1] = (uint64_t)ld_stream(&bn[l + S::T * m + n]);
+      }
+    }
+    sched_fence();
+    r16l_transform<LOG2N, false, 1>(z, z, x, l, ltw);
+    const c64 *tws = launder(twist);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const uint32_t j = l + S::T * S::end_m(s);
+      const c64 r = cmul(untwist_from(tws[j], inv_n), z[s]);
+      if (act) {
+        st_stream(from_torus(r.re), &c[j]);
+        st_stream(from_torus(r.im), &c[j + n]);
+      }
     }
   }
 }
@@ -814,8 +855,12 @@ hipError_t wsub_polymul(uint32_t log2n, const uint64_t *a, const int64_t *b, uin
                              : reinterpret_cast<const void *>(k_wsub_polymul<9>);
   hipError_t e = cfft_launch::ensure_lds(k, kWaveLds);
   if (e != hipSuccess) return e;
+  int cus = 0;
+  e = cu_count(&cus);
+  if (e != hipSuccess) return e;
   const uint64_t per_wg = (uint64_t)kPW * (log2n == 8 ? 4 : 2);
-  const dim3 grid((uint32_t)((batch + per_wg - 1) / per_wg)), block(64 * kPW);
+  const uint64_t want = (batch + per_wg - 1) / per_wg;
+  const dim3 grid((uint32_t)(want < (uint64_t)cus ? want : (uint64_t)cus)), block(64 * kPW);
   const c64 *tb = reinterpret_cast<const c64 *>(tw_blk), *z = reinterpret_cast<const c64 *>(twist);
   if (log2n == 8)
     hipLaunchKernelGGL(k_wsub_polymul<8>, grid, block, kWaveLds, s, a, b, c, batch, tb, z);

@@ -103,17 +103,32 @@ def test_polymul_small_outputs_bit_exact(cfft, oracle):
     assert not c[0].any()
 
 
-@pytest.mark.parametrize("batch", [1, 2, 15, 16, 17, 1001])
+@pytest.mark.parametrize("batch", [1, 2, 15, 16, 17, 1001, 4097, 8195])
 def test_half_wave_products_n1024(cfft, oracle, batch):
     # N = 1024 on a Dif16 plan with base 512: one product per 32-lane
-    # half-wave, 16 per workgroup; odd batches leave the last half idle
+    # half-wave, 16 per workgroup, at most one workgroup per CU (4096
+    # products per round on 256 CUs); odd batches leave the last half idle
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
     N = 1024
     a, b = rand_inputs(batch + 5, batch, N)
     plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 512))
     c = products.negacyclic_polymul_u64_host(plan, a, b)
-    assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 512))
+    assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 512, threads=16))
+
+
+@pytest.mark.parametrize("batch", [1, 3, 33, 8193, 16390])
+def test_quarter_wave_products_n512(cfft, oracle, batch):
+    # N = 512 on a Dif16 plan with base 256: one product per 16-lane
+    # quarter-wave, 32 per workgroup, at most one workgroup per CU (8192
+    # products per round on 256 CUs); the last wave may be partly idle
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 512
+    a, b = rand_inputs(batch + 9, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif16, 256))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    assert np.array_equal(c, oracle.polymul_f64(a, b, 6, 256, threads=16))
 
 
 @pytest.mark.parametrize("batch", [1, 7, 8, 9, 2049])
