@@ -651,8 +651,9 @@ k_wd2_512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
 // 512 h + l + 32 m in its slots m -- so each depth butterfly takes one value
 // from each half-wave: the two halves swap their values through the wave's
 // exchange buffer (slot m of lane i at 64 m + i), and each then forms its own
-// output of the butterfly (a + b in chunk 0; in chunk 1 a - b, written as
-// b' + (-a') with a' its own value, which is the same IEEE operation).  Each
+// output of the butterfly: chunk 0 (own value a) b + a, chunk 1 (own value b)
+// a + (-b) -- the same IEEE results as a + b and a - b, since addition
+// commutes and subtraction is addition of the negated operand.  Each
 // half then runs its chunk's 512-point base transforms (r16l_transform<9>, A
 // and B together, in its own 544-point half of the buffer).  Otherwise the
 // operations of the generic fused product: fold + twist at the loads,
