@@ -178,6 +178,21 @@ def test_base512_product_batches(cfft, oracle, batch):
     assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
 
 
+@pytest.mark.parametrize("batch", [1, 9, 2049, 4100])
+def test_dif8_product_batches(cfft, oracle, batch):
+    """N = 2048 on the Dif8 / 1024 plan (one product per 128-thread
+    workgroup, the register-resident Dif8 schedule): bit-exact against the
+    oracle from one product to a few per CU."""
+    from cfft_amd import products
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    N = 2048
+    a, b = rand_inputs(700 + batch, batch, N)
+    plan = Plan(N // 2, Method.UserProvided(FftAlgo.Dif8, 1024))
+    c = products.negacyclic_polymul_u64_host(plan, a, b)
+    ref = oracle.polymul_f64(a, b, 4, 1024, threads=16)
+    assert np.array_equal(c, ref), (batch, int(np.sum(c != ref)))
+
+
 @pytest.mark.parametrize("N,batch", [(4096, 1), (4096, 1030), (8192, 3), (8192, 515),
                                      (16384, 2), (16384, 259)])
 def test_depth_product_rounds(cfft, oracle, N, batch):
