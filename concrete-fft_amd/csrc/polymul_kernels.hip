@@ -632,12 +632,16 @@ k_wd2_512_transform(c64 *__restrict__ data, uint64_t batch, uint64_t stride,
     vb[m] = ld_stream(&d[512 + l + 32 * m]);
   }
   r16l_depth2_transform<9, FWD>(va, vb, x, l, ltw, tw);
+  // store addresses formed again here rather than kept live across the
+  // transform
+  const uint32_t ls = launder_u32(threadIdx.x) & 31u;
+  c64 *ds = data + (act ? t : batch - 1) * stride;
 #pragma unroll
   for (uint32_t s = 0; s < 16; ++s) {
-    const uint32_t q = l + 32 * S::end_m(s);
+    const uint32_t q = ls + 32 * S::end_m(s);
     if (act) {
-      st_stream(va[s], &d[q]);
-      st_stream(vb[s], &d[512 + q]);
+      st_stream(va[s], &ds[q]);
+      st_stream(vb[s], &ds[512 + q]);
     }
   }
 }
