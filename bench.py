@@ -764,7 +764,10 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
             "traffic": args.traffic_bytes if fresh else None,
-            "kernel": "k_r16_polymul_persistent (fused twist + 2 fwd + pointwise + inv + torus)",
+            "kernel": ("k_r16_polymul_persistent (fused twist + 2 fwd + pointwise + inv + torus)"
+                       if (args.algo, args.base_n) == (6, 1024)
+                       else "fused product kernel of the %s/%d plan"
+                       % (FftAlgo(args.algo).name, args.base_n)),
             "bytes_per_launch": BYTES_PER_PRODUCT * BATCH, "avg_launch_ms": round(avg_ms, 4)}
     if not fresh:
         roof["traffic_note"] = ("kernel sources changed since the HBM counter run: traffic not "
