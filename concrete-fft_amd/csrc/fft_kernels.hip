@@ -167,6 +167,66 @@ __global__ void __launch_bounds__(512) k_polymul_generic(
   run_program<E>(inv, lds, lt, T, tw, load_lds, store_c);
 }
 
+// The same product for one (algo, n, base_n) fixed at compile time: the
+// forward and inverse programs are cfft_prog::fixed_program's, unrolled with
+// constant pass fields (same passes, same operations, same bits).
+template <int ALGO, int LOG2N, int LOG2BN, bool FWD>
+struct FixedProg {
+  static constexpr Program prog = cfft_prog::fixed_program(ALGO, LOG2N, LOG2BN, FWD);
+};
+
+template <int E, int ALGO, int LOG2N, int LOG2BN>
+__global__ void __launch_bounds__(512) k_polymul_fixed(
+    const uint64_t *__restrict__ A, const int64_t *__restrict__ B, uint64_t *__restrict__ C,
+    uint64_t batch, const c64 *__restrict__ tw, const c64 *__restrict__ twist,
+    const c64 *__restrict__ untwist) {
+  extern __shared__ c64 smem[];
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t poly = (uint64_t)blockIdx.x * G + g;
+  const bool active = poly < batch;
+  c64 *lds = smem + g * lds_words(n);
+  constexpr uint64_t N = 2ull * n;
+  const uint64_t *a = A + poly * N;
+  const int64_t *bb = B + poly * N;
+  uint64_t *c = C + poly * N;
+  using Fwd = FixedProg<ALGO, LOG2N, LOG2BN, true>;
+  using Inv = FixedProg<ALGO, LOG2N, LOG2BN, false>;
+
+  auto load_a = [&](uint32_t j) -> c64 {
+    if (!active) return mk(0.0, 0.0);
+    return cmul(twist[j], mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
+  };
+  auto load_b = [&](uint32_t j) -> c64 {
+    if (!active) return mk(0.0, 0.0);
+    return cmul(twist[j], mk(i64_to_f64(bb[j]), i64_to_f64(bb[j + n])));
+  };
+  auto load_lds = [&](uint32_t j) -> c64 { return lds[lds_pad(j)]; };
+  auto store_lds = [&](uint32_t j, c64 v) { lds[lds_pad(j)] = v; };
+  auto store_c = [&](uint32_t j, c64 v) {
+    if (!active) return;
+    c64 r = cmul(untwist[j], v);
+    c[j] = from_torus(r.re);
+    c[j + n] = from_torus(r.im);
+  };
+
+  run_fixed_program<Fwd, 0, E>(lds, lt, T, tw, load_a, store_lds);
+  c64 fa[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) fa[e] = lds[lds_pad(lt + T * (uint32_t)e)];
+  __syncthreads();
+  run_fixed_program<Fwd, 0, E>(lds, lt, T, tw, load_b, store_lds);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const uint32_t j = lds_pad(lt + T * (uint32_t)e);
+    lds[j] = cmul(fa[e], lds[j]);
+  }
+  __syncthreads();
+  run_fixed_program<Inv, 0, E>(lds, lt, T, tw, load_lds, store_c);
+}
+
 // Dynamic LDS above 64 KiB must be opted into per kernel.
 hipError_t set_lds(const void *kernel, size_t bytes) {
   return cfft_launch::ensure_lds(kernel, bytes);
@@ -329,10 +389,56 @@ hipError_t scatter(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, 
   return hipGetLastError();
 }
 
+namespace {
+
+// the (algo, n, base_n) with a compile-time product kernel: n = 1024 plans
+// on the autotuner's bases (512, 1024) without a register-resident kernel
+// and with at most radix 8 (8 points per thread)
+template <int ALGO, int LOG2BN>
+bool try_fixed(const Program &fwd, const Program &inv, const uint64_t *a, const int64_t *b,
+               uint64_t *c, uint64_t batch, const c64 *t, const c64 *tws, const c64 *utw,
+               hipStream_t s, hipError_t *e) {
+  constexpr Program pf = cfft_prog::fixed_program(ALGO, 10, LOG2BN, true);
+  constexpr Program pi = cfft_prog::fixed_program(ALGO, 10, LOG2BN, false);
+  if (!cfft_prog::same_passes(fwd, pf) || !cfft_prog::same_passes(inv, pi)) return false;
+  uint32_t r = 2;
+  for (uint32_t i = 0; i < pf.npass; ++i) r = pf.p[i].radix > r ? pf.p[i].radix : r;
+  const Geom g = geom_for(10, 2048, r);
+  if (g.E != 8) return false;  // only 8-point programs are specialised
+  const uint64_t blocks = (batch + g.G - 1) / g.G;
+  const void *k = reinterpret_cast<const void *>(k_polymul_fixed<8, ALGO, 10, LOG2BN>);
+  *e = set_lds(k, g.lds);
+  if (*e == hipSuccess)
+    hipLaunchKernelGGL((k_polymul_fixed<8, ALGO, 10, LOG2BN>), dim3((uint32_t)blocks),
+                       dim3(g.threads), g.lds, s, a, b, c, batch, t, tws, utw);
+  if (*e == hipSuccess) *e = hipGetLastError();
+  return true;
+}
+
+}  // namespace
+
 hipError_t polymul_generic(const Program &fwd, const Program &inv, const uint64_t *a,
                            const int64_t *b, uint64_t *c, uint64_t batch, const dc64 *tw,
                            const dc64 *twist, const dc64 *untwist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
+  if (fwd.log2n == 10) {
+    const c64 *t = reinterpret_cast<const c64 *>(tw);
+    const c64 *tws = reinterpret_cast<const c64 *>(twist);
+    const c64 *utw = reinterpret_cast<const c64 *>(untwist);
+    hipError_t e = hipSuccess;
+    if (try_fixed<0, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<1, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<2, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<3, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<5, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<0, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<1, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<2, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<3, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<4, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<5, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e))
+      return e;
+  }
   const uint32_t rf = max_radix(fwd), ri = max_radix(inv);
   Geom g = geom_for(fwd.log2n, 2048, rf > ri ? rf : ri);
   const uint64_t blocks = (batch + g.G - 1) / g.G;

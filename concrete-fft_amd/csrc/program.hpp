@@ -48,4 +48,83 @@ struct Program {
   Pass p[kMaxPasses];
 };
 
+// ---- the same pass list, built at compile time ----
+// A plan's whole-transform program (depth passes of src/unordered.rs:
+// 361-367 / 391-489 around the base recursion of src/dif*.rs / src/dit*.rs),
+// for kernels specialised to one (algo, n, base_n): every pass field is then
+// a constant.  The host compares it with the plan's runtime program before
+// using such a kernel.  algo: 0 Dif2, 1 Dit2, 2 Dif4, ..., 7 Dit16.
+constexpr uint32_t c_log2(uint32_t n) {
+  uint32_t l = 0;
+  while ((1u << l) < n) ++l;
+  return l;
+}
+
+constexpr Pass c_pass(uint8_t kind, int radix, bool fwd, uint32_t log2len, uint32_t log2s,
+                      uint32_t log2tw) {
+  Pass p{};
+  p.kind = kind;
+  p.radix = (uint8_t)radix;
+  p.fwd = fwd ? 1 : 0;
+  p.log2len = (uint8_t)log2len;
+  p.s = log2s;
+  p.tw_mul = log2tw;
+  return p;
+}
+
+constexpr Program fixed_program(int algo, uint32_t log2n, uint32_t log2bn, bool fwd) {
+  Program P{};
+  P.log2n = log2n;
+  const int R = 2 << (algo / 2);
+  const uint32_t lr = c_log2((uint32_t)R);
+  const bool dit = (algo & 1) != 0;
+  int radix[8] = {};
+  uint32_t lvl[8] = {};
+  int nlev = 0;
+  for (uint32_t nl = log2n; nl > log2bn; ++nlev) {
+    radix[nlev] = nl == log2bn + 1 ? 2 : nl == log2bn + 2 ? 4 : 8;
+    lvl[nlev] = nl;
+    nl -= c_log2((uint32_t)radix[nlev]);
+  }
+  if (fwd)
+    for (int i = 0; i < nlev; ++i)
+      P.p[P.npass++] = c_pass(DEPTH_FWD, radix[i], true, lvl[i], 0, log2n - lvl[i]);
+  const uint32_t tw = log2n - log2bn;
+  if (log2bn > 0) {
+    uint32_t cores[16] = {};
+    int nc = 0;
+    uint32_t L = log2bn, st = 0;
+    while (L > lr) {
+      cores[nc++] = st;
+      st += lr;
+      L -= lr;
+    }
+    const int rend = 1 << L;
+    if (!dit) {
+      for (int i = 0; i < nc; ++i) P.p[P.npass++] = c_pass(DIF_CORE, R, fwd, log2bn, cores[i], tw);
+      P.p[P.npass++] = c_pass(END, rend, fwd, log2bn, 0, tw);
+    } else {
+      P.p[P.npass++] = c_pass(END, rend, fwd, log2bn, 0, tw);
+      for (int i = nc - 1; i >= 0; --i)
+        P.p[P.npass++] = c_pass(DIT_CORE, R, fwd, log2bn, cores[i], tw);
+    }
+  }
+  if (!fwd)
+    for (int i = nlev - 1; i >= 0; --i)
+      P.p[P.npass++] = c_pass(DEPTH_INV, radix[i], false, lvl[i], 0, log2n - lvl[i]);
+  return P;
+}
+
+// same passes (the fields the executor reads)
+inline bool same_passes(const Program &a, const Program &b) {
+  if (a.npass != b.npass) return false;
+  for (uint32_t i = 0; i < a.npass; ++i) {
+    const Pass &x = a.p[i], &y = b.p[i];
+    if (x.kind != y.kind || x.radix != y.radix || x.fwd != y.fwd || x.log2len != y.log2len ||
+        x.s != y.s || x.tw_mul != y.tw_mul)
+      return false;
+  }
+  return true;
+}
+
 }  // namespace cfft_prog

@@ -143,6 +143,18 @@ __device__ __forceinline__ void run_pass(const Pass &P, bool first, bool last, c
   }
 }
 
+// values the compiler cannot see through, so what is formed from them stays
+// where it is used (an opaque zero offset keeps a pointer's address space)
+__device__ __forceinline__ uint32_t pass_opaque(uint32_t v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ const c64 *pass_opaque(const c64 *p) {
+  uint32_t zero = 0;
+  asm volatile("" : "+s"(zero));
+  return p + zero;
+}
+
 template <int E, class Load, class Store>
 __device__ __forceinline__ void run_program(const Program &prog, c64 *lds, uint32_t lt, uint32_t T,
                                             const c64 *__restrict__ tw, Load &load, Store &store) {
@@ -155,6 +167,23 @@ __device__ __forceinline__ void run_program(const Program &prog, c64 *lds, uint3
       case 8: run_pass<E, 8>(P, first, last, lds, lt, T, tw, load, store); break;
       default: run_pass<E, 16>(P, first, last, lds, lt, T, tw, load, store); break;
     }
+  }
+}
+
+// A program fixed at compile time (FP::prog, a cfft_prog::fixed_program):
+// the passes are unrolled and every pass field is a constant, so positions
+// and twiddle indices fold and no pass descriptor is read at run time.
+template <class FP, int PI, int E, class Load, class Store>
+__device__ __forceinline__ void run_fixed_program(c64 *lds, uint32_t lt, uint32_t T,
+                                                  const c64 *__restrict__ tw, Load &load,
+                                                  Store &store) {
+  if constexpr (PI < (int)FP::prog.npass) {
+    constexpr Pass P = FP::prog.p[PI];
+    // opaque per pass: a pass's addresses and twiddle loads are formed in it,
+    // not hoisted into earlier passes (where they would hold registers)
+    run_pass<E, P.radix>(P, PI == 0, PI + 1 == (int)FP::prog.npass, lds, pass_opaque(lt), T,
+                         pass_opaque(tw), load, store);
+    run_fixed_program<FP, PI + 1, E>(lds, lt, T, tw, load, store);
   }
 }
 

@@ -22,7 +22,11 @@ def torus_err(c, exact):
                                            (1024, 4, 512), (4096, 3, 1024), (256, 0, 128),
                                            (64, 6, 32), (16384, 6, 1024), (2048, 4, 1024),
                                            (4096, 6, 1024), (8192, 6, 1024), (512, 6, 256),
-                                           (2048, 7, 1024), (2048, 5, 1024), (2048, 6, 512)])
+                                           (2048, 7, 1024), (2048, 5, 1024), (2048, 6, 512),
+                                           (2048, 0, 1024), (2048, 1, 1024), (2048, 2, 1024),
+                                           (2048, 3, 1024), (2048, 0, 512), (2048, 1, 512),
+                                           (2048, 2, 512), (2048, 3, 512), (2048, 4, 512),
+                                           (2048, 5, 512)])
 def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
