@@ -227,6 +227,27 @@ __global__ void __launch_bounds__(512) k_polymul_fixed(
   run_fixed_program<Inv, 0, E>(lds, lt, T, tw, load_lds, store_c);
 }
 
+// k_fft_lds for one (algo, n, base_n, direction) fixed at compile time (a
+// whole-transform program: no chunk split).
+template <int E, int ALGO, int LOG2N, int LOG2BN, bool FWD>
+__global__ void __launch_bounds__(512) k_fft_fixed(c64 *__restrict__ data, uint64_t ntrans,
+                                                   uint64_t stride, const c64 *__restrict__ tw) {
+  extern __shared__ c64 smem[];
+  constexpr uint32_t n = 1u << LOG2N;
+  constexpr uint32_t T = n / E;
+  const uint32_t G = blockDim.x / T;
+  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
+  const uint64_t tr = (uint64_t)blockIdx.x * G + g;
+  const bool active = tr < ntrans;
+  c64 *lds = smem + g * lds_words(n);
+  c64 *gp = data + tr * stride;
+  auto load = [&](uint32_t pos) -> c64 { return active ? ld_stream(&gp[pos]) : mk(0.0, 0.0); };
+  auto store = [&](uint32_t pos, c64 v) {
+    if (active) st_stream(v, &gp[pos]);
+  };
+  run_fixed_program<FixedProg<ALGO, LOG2N, LOG2BN, FWD>, 0, E>(lds, lt, T, tw, load, store);
+}
+
 // Dynamic LDS above 64 KiB must be opted into per kernel.
 hipError_t set_lds(const void *kernel, size_t bytes) {
   return cfft_launch::ensure_lds(kernel, bytes);
@@ -264,9 +285,57 @@ Geom geom_for(uint32_t log2n, uint32_t per_wg_points, uint32_t radix) {
 
 namespace cfft_launch {
 
+namespace {
+
+// the whole-transform programs with a compile-time kernel: as for the fused
+// product (n = 1024, bases 512 and 1024, radix <= 8, no register-resident
+// kernel)
+template <int ALGO, int LOG2BN>
+bool try_fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
+                   const dc64 *tw, hipStream_t s, hipError_t *e) {
+  const bool fwd = prog.npass != 0 && prog.p[0].fwd != 0;
+  const Program want = fwd ? cfft_prog::fixed_program(ALGO, 10, LOG2BN, true)
+                           : cfft_prog::fixed_program(ALGO, 10, LOG2BN, false);
+  if (!cfft_prog::same_passes(prog, want)) return false;
+  const Geom g = geom_for(10, 2048, max_radix(prog));
+  if (g.E != 8) return false;
+  const uint64_t blocks = (ntrans + g.G - 1) / g.G;
+  c64 *d = reinterpret_cast<c64 *>(data);
+  const c64 *t = reinterpret_cast<const c64 *>(tw);
+  const void *k = fwd ? reinterpret_cast<const void *>(k_fft_fixed<8, ALGO, 10, LOG2BN, true>)
+                      : reinterpret_cast<const void *>(k_fft_fixed<8, ALGO, 10, LOG2BN, false>);
+  *e = set_lds(k, g.lds);
+  if (*e != hipSuccess) return true;
+  if (fwd)
+    hipLaunchKernelGGL((k_fft_fixed<8, ALGO, 10, LOG2BN, true>), dim3((uint32_t)blocks),
+                       dim3(g.threads), g.lds, s, d, ntrans, stride, t);
+  else
+    hipLaunchKernelGGL((k_fft_fixed<8, ALGO, 10, LOG2BN, false>), dim3((uint32_t)blocks),
+                       dim3(g.threads), g.lds, s, d, ntrans, stride, t);
+  *e = hipGetLastError();
+  return true;
+}
+
+}  // namespace
+
 hipError_t fft_lds(const Program &prog, dc64 *data, uint64_t ntrans, uint32_t chunks_log2,
                    uint64_t stride, const dc64 *tw, hipStream_t s) {
   if (ntrans == 0 || prog.npass == 0) return hipSuccess;
+  if (prog.log2n == 10 && chunks_log2 == 0) {
+    hipError_t e = hipSuccess;
+    if (try_fixed_fft<0, 10>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<1, 10>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<2, 10>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<3, 10>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<5, 10>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<0, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<1, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<2, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<3, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<4, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<5, 9>(prog, data, ntrans, stride, tw, s, &e))
+      return e;
+  }
   Geom g = geom_for(prog.log2n, 2048, max_radix(prog));
   const uint64_t blocks = (ntrans + g.G - 1) / g.G;
   c64 *d = reinterpret_cast<c64 *>(data);
