@@ -298,19 +298,20 @@ bool try_fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t st
                            : cfft_prog::fixed_program(ALGO, 10, LOG2BN, false);
   if (!cfft_prog::same_passes(prog, want)) return false;
   const Geom g = geom_for(10, 2048, max_radix(prog));
-  if (g.E != 8) return false;
+  constexpr int E = ALGO >= 6 ? 16 : 8;
+  if (g.E != E) return false;
   const uint64_t blocks = (ntrans + g.G - 1) / g.G;
   c64 *d = reinterpret_cast<c64 *>(data);
   const c64 *t = reinterpret_cast<const c64 *>(tw);
-  const void *k = fwd ? reinterpret_cast<const void *>(k_fft_fixed<8, ALGO, 10, LOG2BN, true>)
-                      : reinterpret_cast<const void *>(k_fft_fixed<8, ALGO, 10, LOG2BN, false>);
+  const void *k = fwd ? reinterpret_cast<const void *>(k_fft_fixed<E, ALGO, 10, LOG2BN, true>)
+                      : reinterpret_cast<const void *>(k_fft_fixed<E, ALGO, 10, LOG2BN, false>);
   *e = set_lds(k, g.lds);
   if (*e != hipSuccess) return true;
   if (fwd)
-    hipLaunchKernelGGL((k_fft_fixed<8, ALGO, 10, LOG2BN, true>), dim3((uint32_t)blocks),
+    hipLaunchKernelGGL((k_fft_fixed<E, ALGO, 10, LOG2BN, true>), dim3((uint32_t)blocks),
                        dim3(g.threads), g.lds, s, d, ntrans, stride, t);
   else
-    hipLaunchKernelGGL((k_fft_fixed<8, ALGO, 10, LOG2BN, false>), dim3((uint32_t)blocks),
+    hipLaunchKernelGGL((k_fft_fixed<E, ALGO, 10, LOG2BN, false>), dim3((uint32_t)blocks),
                        dim3(g.threads), g.lds, s, d, ntrans, stride, t);
   *e = hipGetLastError();
   return true;
@@ -333,7 +334,8 @@ hipError_t fft_lds(const Program &prog, dc64 *data, uint64_t ntrans, uint32_t ch
         try_fixed_fft<2, 9>(prog, data, ntrans, stride, tw, s, &e) ||
         try_fixed_fft<3, 9>(prog, data, ntrans, stride, tw, s, &e) ||
         try_fixed_fft<4, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<5, 9>(prog, data, ntrans, stride, tw, s, &e))
+        try_fixed_fft<5, 9>(prog, data, ntrans, stride, tw, s, &e) ||
+        try_fixed_fft<7, 9>(prog, data, ntrans, stride, tw, s, &e))
       return e;
   }
   Geom g = geom_for(prog.log2n, 2048, max_radix(prog));
@@ -473,12 +475,13 @@ bool try_fixed(const Program &fwd, const Program &inv, const uint64_t *a, const 
   uint32_t r = 2;
   for (uint32_t i = 0; i < pf.npass; ++i) r = pf.p[i].radix > r ? pf.p[i].radix : r;
   const Geom g = geom_for(10, 2048, r);
-  if (g.E != 8) return false;  // only 8-point programs are specialised
+  constexpr int E = ALGO >= 6 ? 16 : 8;  // radix-16 programs: 16 points per thread
+  if (g.E != E) return false;
   const uint64_t blocks = (batch + g.G - 1) / g.G;
-  const void *k = reinterpret_cast<const void *>(k_polymul_fixed<8, ALGO, 10, LOG2BN>);
+  const void *k = reinterpret_cast<const void *>(k_polymul_fixed<E, ALGO, 10, LOG2BN>);
   *e = set_lds(k, g.lds);
   if (*e == hipSuccess)
-    hipLaunchKernelGGL((k_polymul_fixed<8, ALGO, 10, LOG2BN>), dim3((uint32_t)blocks),
+    hipLaunchKernelGGL((k_polymul_fixed<E, ALGO, 10, LOG2BN>), dim3((uint32_t)blocks),
                        dim3(g.threads), g.lds, s, a, b, c, batch, t, tws, utw);
   if (*e == hipSuccess) *e = hipGetLastError();
   return true;
@@ -505,7 +508,8 @@ hipError_t polymul_generic(const Program &fwd, const Program &inv, const uint64_
         try_fixed<2, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
         try_fixed<3, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
         try_fixed<4, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<5, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e))
+        try_fixed<5, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
+        try_fixed<7, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e))
       return e;
   }
   const uint32_t rf = max_radix(fwd), ri = max_radix(inv);
