@@ -6,6 +6,7 @@
 #include "lds_attr.hpp"
 
 
+#include "fixed_exec.hpp"
 #include "launch.hpp"
 #include "stockham_exec.hpp"
 
@@ -13,6 +14,9 @@ using namespace cfft_dev;
 using cfft_launch::dc64;
 using cfft_prog::Pass;
 using cfft_prog::Program;
+using cfft_fixed::Geom;
+using cfft_fixed::geom_for;
+using cfft_fixed::max_radix;
 
 namespace {
 
@@ -167,118 +171,9 @@ __global__ void __launch_bounds__(512) k_polymul_generic(
   run_program<E>(inv, lds, lt, T, tw, load_lds, store_c);
 }
 
-// The same product for one (algo, n, base_n) fixed at compile time: the
-// forward and inverse programs are cfft_prog::fixed_program's, unrolled with
-// constant pass fields (same passes, same operations, same bits).
-template <int ALGO, int LOG2N, int LOG2BN, bool FWD>
-struct FixedProg {
-  static constexpr Program prog = cfft_prog::fixed_program(ALGO, LOG2N, LOG2BN, FWD);
-};
-
-template <int E, int ALGO, int LOG2N, int LOG2BN>
-__global__ void __launch_bounds__(512) k_polymul_fixed(
-    const uint64_t *__restrict__ A, const int64_t *__restrict__ B, uint64_t *__restrict__ C,
-    uint64_t batch, const c64 *__restrict__ tw, const c64 *__restrict__ twist,
-    const c64 *__restrict__ untwist) {
-  extern __shared__ c64 smem[];
-  constexpr uint32_t n = 1u << LOG2N;
-  constexpr uint32_t T = n / E;
-  const uint32_t G = blockDim.x / T;
-  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
-  const uint64_t poly = (uint64_t)blockIdx.x * G + g;
-  const bool active = poly < batch;
-  c64 *lds = smem + g * lds_words(n);
-  constexpr uint64_t N = 2ull * n;
-  const uint64_t *a = A + poly * N;
-  const int64_t *bb = B + poly * N;
-  uint64_t *c = C + poly * N;
-  using Fwd = FixedProg<ALGO, LOG2N, LOG2BN, true>;
-  using Inv = FixedProg<ALGO, LOG2N, LOG2BN, false>;
-
-  auto load_a = [&](uint32_t j) -> c64 {
-    if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j], mk(torus_to_f64(a[j]), torus_to_f64(a[j + n])));
-  };
-  auto load_b = [&](uint32_t j) -> c64 {
-    if (!active) return mk(0.0, 0.0);
-    return cmul(twist[j], mk(i64_to_f64(bb[j]), i64_to_f64(bb[j + n])));
-  };
-  auto load_lds = [&](uint32_t j) -> c64 { return lds[lds_pad(j)]; };
-  auto store_lds = [&](uint32_t j, c64 v) { lds[lds_pad(j)] = v; };
-  auto store_c = [&](uint32_t j, c64 v) {
-    if (!active) return;
-    c64 r = cmul(untwist[j], v);
-    c[j] = from_torus(r.re);
-    c[j + n] = from_torus(r.im);
-  };
-
-  run_fixed_program<Fwd, 0, E>(lds, lt, T, tw, load_a, store_lds);
-  c64 fa[E];
-#pragma unroll
-  for (int e = 0; e < E; ++e) fa[e] = lds[lds_pad(lt + T * (uint32_t)e)];
-  __syncthreads();
-  run_fixed_program<Fwd, 0, E>(lds, lt, T, tw, load_b, store_lds);
-#pragma unroll
-  for (int e = 0; e < E; ++e) {
-    const uint32_t j = lds_pad(lt + T * (uint32_t)e);
-    lds[j] = cmul(fa[e], lds[j]);
-  }
-  __syncthreads();
-  run_fixed_program<Inv, 0, E>(lds, lt, T, tw, load_lds, store_c);
-}
-
-// k_fft_lds for one (algo, n, base_n, direction) fixed at compile time (a
-// whole-transform program: no chunk split).
-template <int E, int ALGO, int LOG2N, int LOG2BN, bool FWD>
-__global__ void __launch_bounds__(512) k_fft_fixed(c64 *__restrict__ data, uint64_t ntrans,
-                                                   uint64_t stride, const c64 *__restrict__ tw) {
-  extern __shared__ c64 smem[];
-  constexpr uint32_t n = 1u << LOG2N;
-  constexpr uint32_t T = n / E;
-  const uint32_t G = blockDim.x / T;
-  const uint32_t g = threadIdx.x / T, lt = threadIdx.x % T;
-  const uint64_t tr = (uint64_t)blockIdx.x * G + g;
-  const bool active = tr < ntrans;
-  c64 *lds = smem + g * lds_words(n);
-  c64 *gp = data + tr * stride;
-  auto load = [&](uint32_t pos) -> c64 { return active ? ld_stream(&gp[pos]) : mk(0.0, 0.0); };
-  auto store = [&](uint32_t pos, c64 v) {
-    if (active) st_stream(v, &gp[pos]);
-  };
-  run_fixed_program<FixedProg<ALGO, LOG2N, LOG2BN, FWD>, 0, E>(lds, lt, T, tw, load, store);
-}
-
 // Dynamic LDS above 64 KiB must be opted into per kernel.
 hipError_t set_lds(const void *kernel, size_t bytes) {
   return cfft_launch::ensure_lds(kernel, bytes);
-}
-
-// workgroup geometry of the LDS kernels: E points per thread, about 34 KiB of
-// LDS per workgroup (several workgroups per CU) and at most 512 threads.
-// E = 16 only for programs with a radix-16 pass: the 16-point form of the
-// executor needs about twice the registers of the 8-point one.
-struct Geom {
-  int E;
-  uint32_t T, G, threads;
-  size_t lds;
-};
-uint32_t max_radix(const Program &prog) {
-  uint32_t r = 2;
-  for (uint32_t i = 0; i < prog.npass; ++i) r = prog.p[i].radix > r ? prog.p[i].radix : r;
-  return r;
-}
-Geom geom_for(uint32_t log2n, uint32_t per_wg_points, uint32_t radix) {
-  Geom g;
-  const uint32_t n = 1u << log2n;
-  // (8 points per thread up to 4096 points: at most 512 threads per transform)
-  const uint32_t e_max = radix > 8 || n > 4096 ? 16u : 8u;
-  g.E = (int)(n >= e_max ? e_max : n);
-  g.T = n / (uint32_t)g.E;
-  g.G = per_wg_points > n ? per_wg_points / n : 1;
-  if (g.T * g.G > 512) g.G = 512 / g.T > 0 ? 512 / g.T : 1;
-  g.threads = g.T * g.G;
-  g.lds = (size_t)g.G * lds_words(n) * sizeof(c64);
-  return g;
 }
 
 }  // namespace
@@ -287,34 +182,25 @@ namespace cfft_launch {
 
 namespace {
 
-// the whole-transform programs with a compile-time kernel: as for the fused
-// product (n = 1024, bases 512 and 1024, radix <= 8, no register-resident
-// kernel)
-template <int ALGO, int LOG2BN>
-bool try_fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
-                   const dc64 *tw, hipStream_t s, hipError_t *e) {
-  const bool fwd = prog.npass != 0 && prog.p[0].fwd != 0;
-  const Program want = fwd ? cfft_prog::fixed_program(ALGO, 10, LOG2BN, true)
-                           : cfft_prog::fixed_program(ALGO, 10, LOG2BN, false);
-  if (!cfft_prog::same_passes(prog, want)) return false;
-  const Geom g = geom_for(10, 2048, max_radix(prog));
-  constexpr int E = ALGO >= 6 ? 16 : 8;
-  if (g.E != E) return false;
-  const uint64_t blocks = (ntrans + g.G - 1) / g.G;
-  c64 *d = reinterpret_cast<c64 *>(data);
-  const c64 *t = reinterpret_cast<const c64 *>(tw);
-  const void *k = fwd ? reinterpret_cast<const void *>(k_fft_fixed<E, ALGO, 10, LOG2BN, true>)
-                      : reinterpret_cast<const void *>(k_fft_fixed<E, ALGO, 10, LOG2BN, false>);
-  *e = set_lds(k, g.lds);
-  if (*e != hipSuccess) return true;
-  if (fwd)
-    hipLaunchKernelGGL((k_fft_fixed<E, ALGO, 10, LOG2BN, true>), dim3((uint32_t)blocks),
-                       dim3(g.threads), g.lds, s, d, ntrans, stride, t);
-  else
-    hipLaunchKernelGGL((k_fft_fixed<E, ALGO, 10, LOG2BN, false>), dim3((uint32_t)blocks),
-                       dim3(g.threads), g.lds, s, d, ntrans, stride, t);
-  *e = hipGetLastError();
-  return true;
+bool fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
+               const dc64 *tw, hipStream_t s, hipError_t *e) {
+  switch (prog.log2n) {
+    case 9: return fixed_fft_n512(prog, data, ntrans, stride, tw, s, e);
+    case 10: return fixed_fft_n1024(prog, data, ntrans, stride, tw, s, e);
+    case 11: return fixed_fft_n2048(prog, data, ntrans, stride, tw, s, e);
+    default: return false;
+  }
+}
+
+bool fixed_polymul(const Program &fwd, const Program &inv, const uint64_t *a, const int64_t *b,
+                   uint64_t *c, uint64_t batch, const dc64 *tw, const dc64 *twist,
+                   const dc64 *untwist, hipStream_t s, hipError_t *e) {
+  switch (fwd.log2n) {
+    case 9: return fixed_polymul_n512(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
+    case 10: return fixed_polymul_n1024(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
+    case 11: return fixed_polymul_n2048(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
+    default: return false;
+  }
 }
 
 }  // namespace
@@ -322,21 +208,9 @@ bool try_fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t st
 hipError_t fft_lds(const Program &prog, dc64 *data, uint64_t ntrans, uint32_t chunks_log2,
                    uint64_t stride, const dc64 *tw, hipStream_t s) {
   if (ntrans == 0 || prog.npass == 0) return hipSuccess;
-  if (prog.log2n == 10 && chunks_log2 == 0) {
+  if (chunks_log2 == 0) {
     hipError_t e = hipSuccess;
-    if (try_fixed_fft<0, 10>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<1, 10>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<2, 10>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<3, 10>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<5, 10>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<0, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<1, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<2, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<3, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<4, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<5, 9>(prog, data, ntrans, stride, tw, s, &e) ||
-        try_fixed_fft<7, 9>(prog, data, ntrans, stride, tw, s, &e))
-      return e;
+    if (fixed_fft(prog, data, ntrans, stride, tw, s, &e)) return e;
   }
   Geom g = geom_for(prog.log2n, 2048, max_radix(prog));
   const uint64_t blocks = (ntrans + g.G - 1) / g.G;
@@ -460,57 +334,13 @@ hipError_t scatter(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, 
   return hipGetLastError();
 }
 
-namespace {
-
-// the (algo, n, base_n) with a compile-time product kernel: n = 1024 plans
-// on the autotuner's bases (512, 1024) without a register-resident kernel
-// and with at most radix 8 (8 points per thread)
-template <int ALGO, int LOG2BN>
-bool try_fixed(const Program &fwd, const Program &inv, const uint64_t *a, const int64_t *b,
-               uint64_t *c, uint64_t batch, const c64 *t, const c64 *tws, const c64 *utw,
-               hipStream_t s, hipError_t *e) {
-  constexpr Program pf = cfft_prog::fixed_program(ALGO, 10, LOG2BN, true);
-  constexpr Program pi = cfft_prog::fixed_program(ALGO, 10, LOG2BN, false);
-  if (!cfft_prog::same_passes(fwd, pf) || !cfft_prog::same_passes(inv, pi)) return false;
-  uint32_t r = 2;
-  for (uint32_t i = 0; i < pf.npass; ++i) r = pf.p[i].radix > r ? pf.p[i].radix : r;
-  const Geom g = geom_for(10, 2048, r);
-  constexpr int E = ALGO >= 6 ? 16 : 8;  // radix-16 programs: 16 points per thread
-  if (g.E != E) return false;
-  const uint64_t blocks = (batch + g.G - 1) / g.G;
-  const void *k = reinterpret_cast<const void *>(k_polymul_fixed<E, ALGO, 10, LOG2BN>);
-  *e = set_lds(k, g.lds);
-  if (*e == hipSuccess)
-    hipLaunchKernelGGL((k_polymul_fixed<E, ALGO, 10, LOG2BN>), dim3((uint32_t)blocks),
-                       dim3(g.threads), g.lds, s, a, b, c, batch, t, tws, utw);
-  if (*e == hipSuccess) *e = hipGetLastError();
-  return true;
-}
-
-}  // namespace
-
 hipError_t polymul_generic(const Program &fwd, const Program &inv, const uint64_t *a,
                            const int64_t *b, uint64_t *c, uint64_t batch, const dc64 *tw,
                            const dc64 *twist, const dc64 *untwist, hipStream_t s) {
   if (batch == 0) return hipSuccess;
-  if (fwd.log2n == 10) {
-    const c64 *t = reinterpret_cast<const c64 *>(tw);
-    const c64 *tws = reinterpret_cast<const c64 *>(twist);
-    const c64 *utw = reinterpret_cast<const c64 *>(untwist);
+  {
     hipError_t e = hipSuccess;
-    if (try_fixed<0, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<1, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<2, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<3, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<5, 10>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<0, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<1, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<2, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<3, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<4, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<5, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e) ||
-        try_fixed<7, 9>(fwd, inv, a, b, c, batch, t, tws, utw, s, &e))
-      return e;
+    if (fixed_polymul(fwd, inv, a, b, c, batch, tw, twist, untwist, s, &e)) return e;
   }
   const uint32_t rf = max_radix(fwd), ri = max_radix(inv);
   Geom g = geom_for(fwd.log2n, 2048, rf > ri ? rf : ri);

@@ -42,7 +42,31 @@ hipError_t gather(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, u
 hipError_t scatter(const dc64 *src, dc64 *dst, const uint32_t *idx, uint32_t n, uint64_t batch,
                    hipStream_t s);
 
-// Fused negacyclic product over the generic pass executor.
+// Compile-time pass programs (fixed_exec.hpp): launch the plan's product /
+// whole-transform program on a kernel specialised to it and return true, or
+// return false (nothing launched) when no such kernel is built for it.
+// *e: the launch status.
+bool fixed_polymul_n512(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                        const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                        const dc64 *tw, const dc64 *twist, const dc64 *untwist, hipStream_t s,
+                        hipError_t *e);
+bool fixed_polymul_n1024(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                         const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                         const dc64 *tw, const dc64 *twist, const dc64 *untwist, hipStream_t s,
+                         hipError_t *e);
+bool fixed_polymul_n2048(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                         const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                         const dc64 *tw, const dc64 *twist, const dc64 *untwist, hipStream_t s,
+                         hipError_t *e);
+bool fixed_fft_n512(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
+                    const dc64 *tw, hipStream_t s, hipError_t *e);
+bool fixed_fft_n1024(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,
+                     uint64_t stride, const dc64 *tw, hipStream_t s, hipError_t *e);
+bool fixed_fft_n2048(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,
+                     uint64_t stride, const dc64 *tw, hipStream_t s, hipError_t *e);
+
+// Fused negacyclic product over the generic pass executor (a compile-time
+// program's kernel where one is built).
 hipError_t polymul_generic(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
                            const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
                            const dc64 *tw, const dc64 *twist, const dc64 *untwist,

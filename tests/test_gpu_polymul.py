@@ -26,7 +26,11 @@ def torus_err(c, exact):
                                            (2048, 0, 1024), (2048, 1, 1024), (2048, 2, 1024),
                                            (2048, 3, 1024), (2048, 0, 512), (2048, 1, 512),
                                            (2048, 2, 512), (2048, 3, 512), (2048, 4, 512),
-                                           (2048, 5, 512)])
+                                           (2048, 5, 512), (1024, 0, 512), (1024, 1, 512),
+                                           (1024, 2, 512), (1024, 3, 512), (1024, 5, 512),
+                                           (1024, 7, 512), (4096, 0, 1024), (4096, 5, 1024),
+                                           (4096, 7, 1024), (4096, 2, 512), (4096, 4, 512),
+                                           (4096, 6, 512), (4096, 7, 512)])
 def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
