@@ -4,8 +4,8 @@
 // geometry shared with the runtime executor's kernels.  With every pass
 // field a constant the passes unroll, positions and twiddle indices fold and
 // no pass descriptor is read at run time; the operations, and so the bits,
-// are the runtime executor's.  fixed_n512.hip, fixed_n1024.hip and
-// fixed_n2048.hip instantiate them for the plans listed there.
+// are the runtime executor's.  fixed_n512.hip ... fixed_n4096.hip instantiate
+// them for the plans listed there.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -147,7 +147,8 @@ bool fixed_polymul_try(const Program &fwd, const Program &inv, const uint64_t *a
   if (!cfft_prog::same_passes(fwd, pf) || !cfft_prog::same_passes(inv, pi)) return false;
   const uint32_t rf = cfft_fixed::max_radix(pf), ri = cfft_fixed::max_radix(pi);
   const cfft_fixed::Geom g = cfft_fixed::geom_for(LOG2N, 2048, rf > ri ? rf : ri);
-  constexpr int E = ALGO >= 6 ? 16 : 8;  // radix-16 programs: 16 points per thread
+  // 16 points per thread for radix-16 programs and above 4096 points
+  constexpr int E = ALGO >= 6 || LOG2N > 12 ? 16 : 8;
   if (g.E != E) return false;
   const uint64_t blocks = (batch + g.G - 1) / g.G;
   const void *k = reinterpret_cast<const void *>(k_polymul_fixed<E, ALGO, LOG2N, LOG2BN>);
@@ -167,7 +168,7 @@ bool fixed_fft_try(const Program &prog, c64 *d, uint64_t ntrans, uint64_t stride
                            : cfft_prog::fixed_program(ALGO, LOG2N, LOG2BN, false);
   if (!cfft_prog::same_passes(prog, want)) return false;
   const cfft_fixed::Geom g = cfft_fixed::geom_for(LOG2N, 2048, cfft_fixed::max_radix(prog));
-  constexpr int E = ALGO >= 6 ? 16 : 8;
+  constexpr int E = ALGO >= 6 || LOG2N > 12 ? 16 : 8;
   if (g.E != E) return false;
   const uint64_t blocks = (ntrans + g.G - 1) / g.G;
   const void *k = fwd ? reinterpret_cast<const void *>(k_fft_fixed<E, ALGO, LOG2N, LOG2BN, true>)
