@@ -608,7 +608,8 @@ def bench_matrix(steps=100):
     return {"workload": "reference bench matrix on device buffers (batch-1 device time per "
                         "call; batched: 8 Mi c64 / 4 Mi f128 complex points per call); fused "
                         "products: 16 Mi coefficients per operand per call",
-            "unordered": rows, "fft128": r128, "products": bench_products()}
+            "unordered": rows, "fft128": r128, "products": bench_products(),
+            "plans": bench_plans()}
 
 
 def bench_products(steps=20):
@@ -638,6 +639,43 @@ def bench_products(steps=20):
     return prods
 
 
+def bench_plans(steps=20):
+    """Every f64 plan the autotuner considers at N = 2048 (8 algorithms x
+    base_n 512 / 1024): fused products/s at batch 8192 and batched n = 1024
+    forward / inverse transforms/s at batch 8192, each with the kernel kind
+    the plan runs on (register-resident, compile-time pass program)."""
+    from cfft_amd import check, lib
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    s = Stream()
+    rng = np.random.default_rng(12)
+    N, batch = 2048, 8192
+    n = N // 2
+    a = DeviceBuffer.from_host(rng.integers(0, 1 << 64, batch * N, dtype=np.uint64), s)
+    b = DeviceBuffer.from_host(rng.integers(-128, 128, batch * N, dtype=np.int64), s)
+    c = DeviceBuffer((batch * N,), np.uint64)
+    x = DeviceBuffer.from_host(rng.uniform(-1, 1, (batch, 2 * n)).view(np.complex128), s)
+    register = {("Dif16", 1024), ("Dit16", 1024), ("Dif8", 1024), ("Dif16", 512)}
+    out = {}
+    for base in (1024, 512):
+        for algo in FftAlgo:
+            plan = Plan(n, Method.UserProvided(algo, base))
+            mp = time_region(lambda: check(lib.cfft_negacyclic_polymul_u64(
+                plan.handle, a.dptr, b.dptr, c.dptr, batch, s.handle)), s, steps, 3)
+            mf = time_region(lambda: check(lib.cfft_fwd(plan.handle, x.dptr, batch, n, s.handle)),
+                             s, steps, 3)
+            mi = time_region(lambda: check(lib.cfft_inv(plan.handle, x.dptr, batch, n, s.handle)),
+                             s, steps, 3)
+            out["%s/%d" % (algo.name, base)] = {
+                "kernel": "register-resident" if (algo.name, base) in register
+                          else "compile-time pass program",
+                "products_per_s": round(batch / (mp * 1e-3), 1),
+                "fwd_per_s": round(batch / (mf * 1e-3), 1),
+                "inv_per_s": round(batch / (mi * 1e-3), 1)}
+    return {"workload": "N = 2048 products and n = 1024 transforms, batch 8192 each",
+            "plans": out}
+
+
 def copy_probe(mib):
     from cfft_amd import check, lib
     from cfft_amd.device import DeviceBuffer, Stream
@@ -664,7 +702,8 @@ def main():
     ap.add_argument("--traffic-bytes", type=float, default=TRAFFIC_BYTES,
                     help="HBM bytes per launch of the product kernel (rocprofv3 --pmc)")
     ap.add_argument("--mode", choices=["all", "polymul", "uneven", "serve", "exact", "ordered", "products",
-                                       "transforms", "matrix", "host", "copy"], default="all")
+                                       "plans", "transforms", "matrix", "host", "copy"],
+                    default="all")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--uneven-steps", type=int, default=20)
     ap.add_argument("--exact-steps", type=int, default=50)
@@ -710,6 +749,9 @@ def main():
         return 0
     if args.mode == "products":
         print(json.dumps({"mode": "products", "products": bench_products()}))
+        return 0
+    if args.mode == "plans":
+        print(json.dumps({"mode": "plans", "plans": bench_plans()}))
         return 0
     if args.mode == "host":
         print(json.dumps({"mode": "host", "host_path": bench_host_path()}))
