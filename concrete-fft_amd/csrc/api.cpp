@@ -88,75 +88,23 @@ int algo_radix(int algo) {
     default: return 0;
   }
 }
-bool algo_is_dit(int algo) {
-  return algo == CFFT_DIT2 || algo == CFFT_DIT4 || algo == CFFT_DIT8 || algo == CFFT_DIT16;
-}
 const char *algo_name(int algo) {
   static const char *names[] = {"Dif2", "Dit2", "Dif4", "Dit4", "Dif8", "Dit8", "Dif16", "Dit16"};
   return (algo >= 0 && algo < 8) ? names[algo] : "?";
 }
 
-Pass mkpass(uint8_t kind, int radix, bool fwd, uint32_t log2len, uint32_t log2s, uint32_t log2tw) {
-  Pass p{};
-  p.kind = kind;
-  p.radix = (uint8_t)radix;
-  p.fwd = fwd ? 1 : 0;
-  p.log2len = (uint8_t)log2len;
-  p.s = log2s;
-  p.tw_mul = log2tw;
-  return p;
-}
-
-// Stockham recursion of src/dif{2,4,8,16}.rs / src/dit{2,4,8,16}.rs as a
-// pass list: core passes while more than log2(R) levels remain, then the end
-// butterfly of radix 2^L; DIT runs the end first and the cores in reverse.
-void base_passes(std::vector<Pass> &out, int algo, uint32_t log2bn, bool fwd, uint32_t log2n) {
-  const int R = algo_radix(algo);
-  const uint32_t lr = ilog2((size_t)R);
-  const uint32_t tw = log2n - log2bn;
-  uint32_t L = log2bn;
-  if (L == 0) return;  // n == 1: do_nothing (src/ordered.rs:204)
-  std::vector<uint32_t> cores;
-  uint32_t s = 0;
-  while (L > lr) {
-    cores.push_back(s);
-    s += lr;
-    L -= lr;
-  }
-  const int rend = 1 << L;
-  if (!algo_is_dit(algo)) {
-    for (uint32_t cs : cores) out.push_back(mkpass(cfft_prog::DIF_CORE, R, fwd, log2bn, cs, tw));
-    out.push_back(mkpass(cfft_prog::END, rend, fwd, log2bn, 0, tw));
-  } else {
-    out.push_back(mkpass(cfft_prog::END, rend, fwd, log2bn, 0, tw));
-    for (auto it = cores.rbegin(); it != cores.rend(); ++it)
-      out.push_back(mkpass(cfft_prog::DIT_CORE, R, fwd, log2bn, *it, tw));
-  }
-}
-
-// depth radix choice, src/unordered.rs:361-367 and :407-413
-int depth_radix(size_t n, size_t base_n) {
-  if (n == 2 * base_n) return 2;
-  if (n == 4 * base_n) return 4;
-  return 8;
-}
-
-void full_program(std::vector<Pass> &fwd, std::vector<Pass> &inv, size_t n, size_t base_n,
+// The plan's pass lists: cfft_prog::fixed_program (program.hpp), the same
+// builder the compile-time kernels use, so their programs match by
+// construction.
+bool full_program(std::vector<Pass> &fwd, std::vector<Pass> &inv, size_t n, size_t base_n,
                   int algo) {
-  const uint32_t log2n = ilog2(n), log2bn = ilog2(base_n);
-  std::vector<std::pair<int, uint32_t>> levels;  // (radix, log2 len), top first
-  size_t nl = n;
-  while (nl > base_n) {
-    const int r = depth_radix(nl, base_n);
-    levels.push_back({r, ilog2(nl)});
-    nl /= (size_t)r;
-  }
-  for (auto &lv : levels)
-    fwd.push_back(mkpass(cfft_prog::DEPTH_FWD, lv.first, true, lv.second, 0, log2n - lv.second));
-  base_passes(fwd, algo, log2bn, true, log2n);
-  base_passes(inv, algo, log2bn, false, log2n);
-  for (auto it = levels.rbegin(); it != levels.rend(); ++it)
-    inv.push_back(mkpass(cfft_prog::DEPTH_INV, it->first, false, it->second, 0, log2n - it->second));
+  const Program f = cfft_prog::fixed_program(algo, ilog2(n), ilog2(base_n), true);
+  const Program i = cfft_prog::fixed_program(algo, ilog2(n), ilog2(base_n), false);
+  if (f.npass > (uint32_t)cfft_prog::kMaxPasses || i.npass > (uint32_t)cfft_prog::kMaxPasses)
+    return false;
+  fwd.assign(f.p, f.p + f.npass);
+  inv.assign(i.p, i.p + i.npass);
+  return true;
 }
 
 bool to_program(const std::vector<Pass> &v, size_t first, size_t last, Program &P) {
@@ -176,7 +124,8 @@ int build_f64_plan(cfft_plan_s *p) {
   const size_t n = p->n;
   const uint32_t log2n = ilog2(n);
   std::vector<Pass> fwd, inv;
-  full_program(fwd, inv, n, p->base_n, p->algo);
+  if (!full_program(fwd, inv, n, p->base_n, p->algo))
+    return set_error(CFFT_ERROR_SIZE, "transform needs more passes than the device program holds");
   // whole-transform programs for fused kernels
   p->fwd_full = Program{};
   p->inv_full = Program{};

@@ -48,12 +48,17 @@ struct Program {
   Pass p[kMaxPasses];
 };
 
-// ---- the same pass list, built at compile time ----
-// A plan's whole-transform program (depth passes of src/unordered.rs:
-// 361-367 / 391-489 around the base recursion of src/dif*.rs / src/dit*.rs),
-// for kernels specialised to one (algo, n, base_n): every pass field is then
-// a constant.  The host compares it with the plan's runtime program before
-// using such a kernel.  algo: 0 Dif2, 1 Dit2, 2 Dif4, ..., 7 Dit16.
+// ---- building a plan's pass lists ----
+// A plan's whole-transform programs: the depth levels above base_n (radix 2
+// when n = 2 base_n, 4 when n = 4 base_n, else 8: src/unordered.rs:361-367,
+// :407-413; forward top level first, inverse in reverse) around the base
+// recursion of src/dif{2,4,8,16}.rs / src/dit{2,4,8,16}.rs as a pass list
+// (core passes while more than log2(R) levels remain, then the end butterfly
+// of radix 2^L; DIT runs the end first and the cores in reverse).  constexpr,
+// so the plans are built with it at run time and kernels specialised to one
+// (algo, n, base_n) use it at compile time (every pass field a constant);
+// their launchers still compare the two before such a launch.  algo: 0 Dif2,
+// 1 Dit2, 2 Dif4, ..., 7 Dit16.
 constexpr uint32_t c_log2(uint32_t n) {
   uint32_t l = 0;
   while ((1u << l) < n) ++l;
@@ -75,43 +80,56 @@ constexpr Pass c_pass(uint8_t kind, int radix, bool fwd, uint32_t log2len, uint3
 constexpr Program fixed_program(int algo, uint32_t log2n, uint32_t log2bn, bool fwd) {
   Program P{};
   P.log2n = log2n;
+  // a list longer than kMaxPasses is cut there (npass = kMaxPasses + 1
+  // marks it); plans up to n = 2^30 with base_n <= 1024 take at most 20
+  bool over = false;
+  auto add = [&](Pass q) {
+    if (P.npass < (uint32_t)kMaxPasses) P.p[P.npass++] = q;
+    else over = true;
+  };
   const int R = 2 << (algo / 2);
   const uint32_t lr = c_log2((uint32_t)R);
   const bool dit = (algo & 1) != 0;
-  int radix[8] = {};
-  uint32_t lvl[8] = {};
+  int radix[16] = {};
+  uint32_t lvl[16] = {};
   int nlev = 0;
   for (uint32_t nl = log2n; nl > log2bn; ++nlev) {
+    if (nlev == 16) {  // beyond any supported size
+      over = true;
+      break;
+    }
     radix[nlev] = nl == log2bn + 1 ? 2 : nl == log2bn + 2 ? 4 : 8;
     lvl[nlev] = nl;
     nl -= c_log2((uint32_t)radix[nlev]);
   }
   if (fwd)
     for (int i = 0; i < nlev; ++i)
-      P.p[P.npass++] = c_pass(DEPTH_FWD, radix[i], true, lvl[i], 0, log2n - lvl[i]);
+      add(c_pass(DEPTH_FWD, radix[i], true, lvl[i], 0, log2n - lvl[i]));
   const uint32_t tw = log2n - log2bn;
   if (log2bn > 0) {
     uint32_t cores[16] = {};
     int nc = 0;
     uint32_t L = log2bn, st = 0;
-    while (L > lr) {
+    while (L > lr && nc < 16) {
       cores[nc++] = st;
       st += lr;
       L -= lr;
     }
+    if (L > lr) over = true;  // beyond any supported size
     const int rend = 1 << L;
     if (!dit) {
-      for (int i = 0; i < nc; ++i) P.p[P.npass++] = c_pass(DIF_CORE, R, fwd, log2bn, cores[i], tw);
-      P.p[P.npass++] = c_pass(END, rend, fwd, log2bn, 0, tw);
+      for (int i = 0; i < nc; ++i) add(c_pass(DIF_CORE, R, fwd, log2bn, cores[i], tw));
+      add(c_pass(END, rend, fwd, log2bn, 0, tw));
     } else {
-      P.p[P.npass++] = c_pass(END, rend, fwd, log2bn, 0, tw);
+      add(c_pass(END, rend, fwd, log2bn, 0, tw));
       for (int i = nc - 1; i >= 0; --i)
-        P.p[P.npass++] = c_pass(DIT_CORE, R, fwd, log2bn, cores[i], tw);
+        add(c_pass(DIT_CORE, R, fwd, log2bn, cores[i], tw));
     }
   }
   if (!fwd)
     for (int i = nlev - 1; i >= 0; --i)
-      P.p[P.npass++] = c_pass(DEPTH_INV, radix[i], false, lvl[i], 0, log2n - lvl[i]);
+      add(c_pass(DEPTH_INV, radix[i], false, lvl[i], 0, log2n - lvl[i]));
+  if (over) P.npass = kMaxPasses + 1;
   return P;
 }
 
