@@ -1,7 +1,8 @@
 // fixed_exec.hpp -- kernels that run a plan's pass program fixed at compile
-// time (cfft_prog::fixed_program: the same passes as the plan's runtime
-// program, which the launchers compare before use), and the workgroup
-// geometry shared with the runtime executor's kernels.  With every pass
+// time (cfft_prog::fixed_program, the builder the plans are made with; each
+// launcher checks that the plan's program is the one it was instantiated
+// for), and the workgroup geometry shared with the runtime executor's
+// kernels.  With every pass
 // field a constant the passes unroll, positions and twiddle indices fold and
 // no pass descriptor is read at run time; the operations, and so the bits,
 // are the runtime executor's.  fixed_n512.hip ... fixed_n4096.hip instantiate
