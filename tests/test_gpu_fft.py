@@ -182,6 +182,34 @@ def test_device_path_with_stride(cfft, oracle, n, stride, batch, base):
     assert np.array_equal(w[:, n:], x[:, n:])
 
 
+@pytest.mark.parametrize("n,stride,batch,algo,base", [(512, 530, 9, 2, 512), (1024, 1100, 7, 5, 1024),
+                                                      (1024, 1040, 5, 7, 512), (2048, 2050, 5, 0, 1024),
+                                                      (4096, 4100, 3, 3, 512)])
+def test_compile_time_programs_with_stride(cfft, oracle, n, stride, batch, algo, base):
+    """Plans run by the compile-time pass programs (Dif4/512, Dit8/1024,
+    Dit16/512, Dif2/1024, Dit4/512): strided batches in place through the C
+    ABI, bit-exact against the oracle, padding between transforms untouched."""
+    import cfft_amd
+    from cfft_amd.device import DeviceBuffer, Stream
+    from cfft_amd.unordered import FftAlgo, Method, Plan
+    rng = np.random.default_rng(n + algo)
+    x = rand_c(rng, (batch, stride))
+    plan = Plan(n, Method.UserProvided(FftAlgo(algo), base))
+    s = Stream()
+    d = DeviceBuffer.from_host(x)
+    cfft_amd.check(cfft_amd.lib.cfft_fwd(plan.handle, d.dptr, batch, stride, s.handle))
+    s.synchronize()
+    y = d.download()
+    ref = oracle.unordered_fwd(np.ascontiguousarray(x[:, :n]), algo, base)
+    assert np.array_equal(y[:, :n], ref), (n, algo, base)
+    assert np.array_equal(y[:, n:], x[:, n:]), "padding between transforms was touched"
+    cfft_amd.check(cfft_amd.lib.cfft_inv(plan.handle, d.dptr, batch, stride, s.handle))
+    s.synchronize()
+    w = d.download()
+    assert np.array_equal(w[:, :n], oracle.unordered_inv(ref, algo, base)), (n, algo, base)
+    assert np.array_equal(w[:, n:], x[:, n:])
+
+
 @pytest.mark.parametrize("n", [256, 512, 1024])
 def test_fwd_monomial(cfft, oracle, n):
     # test_fwd_monomial (src/unordered.rs:1108-1137): fwd_monomial(d) == fwd(e_d)
