@@ -189,6 +189,7 @@ bool fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride
     case 10: return fixed_fft_n1024(prog, data, ntrans, stride, tw, s, e);
     case 11: return fixed_fft_n2048(prog, data, ntrans, stride, tw, s, e);
     case 12: return fixed_fft_n4096(prog, data, ntrans, stride, tw, s, e);
+    case 13: return fixed_fft_n8192(prog, data, ntrans, stride, tw, s, e);
     default: return false;
   }
 }
@@ -201,6 +202,7 @@ bool fixed_polymul(const Program &fwd, const Program &inv, const uint64_t *a, co
     case 10: return fixed_polymul_n1024(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     case 11: return fixed_polymul_n2048(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     case 12: return fixed_polymul_n4096(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
+    case 13: return fixed_polymul_n8192(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     default: return false;
   }
 }

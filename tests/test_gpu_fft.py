@@ -184,10 +184,10 @@ def test_device_path_with_stride(cfft, oracle, n, stride, batch, base):
 
 @pytest.mark.parametrize("n,stride,batch,algo,base", [(512, 530, 9, 2, 512), (1024, 1100, 7, 5, 1024),
                                                       (1024, 1040, 5, 7, 512), (2048, 2050, 5, 0, 1024),
-                                                      (4096, 4100, 3, 3, 512)])
+                                                      (4096, 4100, 3, 3, 512), (8192, 8200, 2, 5, 1024)])
 def test_compile_time_programs_with_stride(cfft, oracle, n, stride, batch, algo, base):
     """Plans run by the compile-time pass programs (Dif4/512, Dit8/1024,
-    Dit16/512, Dif2/1024, Dit4/512): strided batches in place through the C
+    Dit16/512, Dif2/1024, Dit4/512, Dit8/1024 at n = 8192): strided batches in place through the C
     ABI, bit-exact against the oracle, padding between transforms untouched."""
     import cfft_amd
     from cfft_amd.device import DeviceBuffer, Stream
