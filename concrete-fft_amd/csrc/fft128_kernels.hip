@@ -482,11 +482,16 @@ hipError_t set_lds(const void *kernel, size_t bytes) {
 
 namespace cfft_launch {
 
-// Largest fft128 block held in one workgroup: 4096 points = 136 KiB LDS.
-// Larger transforms (up to 2^kFft128MaxLog2) run their stages above the
-// block over global memory: forward first (CT runs from the top bit down),
-// inverse last (GS runs from the bottom bit up).
-constexpr uint32_t kFft128BlockLog2 = 12;
+// fft128 block held in one workgroup: 2048 points = 68 KiB of LDS, so two
+// workgroups share a CU (a 4096-point block, 136 KiB, leaves one per CU
+// waiting at every barrier; measured, batched n = 8192 forward 2.0 -> 2.7 M/s,
+// n = 16384 0.92 -> 1.19 M/s, n = 4096 forward 4.4 -> 5.2 M/s), except the
+// n = 4096 inverse, which keeps the whole transform in one block (5.2 against
+// 5.0 M/s).  Larger transforms (up to 2^kFft128MaxLog2) run their stages
+// above the block over global memory: forward first (CT runs from the top
+// bit down), inverse last (GS runs from the bottom bit up).
+constexpr uint32_t kFft128BlockLog2 = 11;
+constexpr uint32_t kFft128InvWholeLog2 = 12;
 constexpr uint32_t kFft128MaxLog2 = 20;
 
 namespace {
@@ -536,7 +541,9 @@ template <bool FWD, int MULV>
 hipError_t fft128_any(uint32_t Lg, double *re0, double *re1, double *im0, double *im1,
                       const double *t_re0, const double *t_re1, const double *t_im0,
                       const double *t_im1, uint64_t batch, hipStream_t s) {
-  const uint32_t L = Lg < kFft128BlockLog2 ? Lg : kFft128BlockLog2;
+  const uint32_t L = !FWD && Lg == kFft128InvWholeLog2 ? Lg
+                     : Lg < kFft128BlockLog2             ? Lg
+                                                         : kFft128BlockLog2;
   hipError_t e = hipSuccess;
   if (FWD) {
     // bits Lg-1 .. L in rounds of up to 3 from the top, then the blocks

@@ -131,10 +131,11 @@ int cfft_ordered_plan_create_large(size_t n, int method, int algo, double measur
 int cfft_unordered_plan_create(size_t n, int method, int base_algo, size_t base_n,
                                double measure_seconds, cfft_plan *out);
 /* fft128::Plan::new(n), src/fft128/mod.rs:1864-1885: n a power of two,
- * n >= 32 (and n <= 2^20 on the device).  Up to 4096 points a transform
- * runs in one workgroup's LDS; above, the stages above the 4096-point
- * blocks run over global memory (before the blocks forward, after them
- * inverse), with the same butterflies and so the same bits. */
+ * n >= 32 (and n <= 2^20 on the device).  Up to 2048 points (and the
+ * 4096-point inverse) a transform runs in one workgroup's LDS; above, the
+ * stages above the 2048-point blocks run over global memory (before the
+ * blocks forward, after them inverse), with the same butterflies and so
+ * the same bits. */
 int cfft128_plan_create(size_t n, cfft_plan *out);
 int cfft_plan_destroy(cfft_plan plan);
 
