@@ -29,7 +29,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
 
 HIP_SOURCES = ["fft_kernels.hip", "polymul_kernels.hip", "polymul_n2048_kernels.hip", "fft128_kernels.hip",
                "exact_kernels.hip", "extprod_kernels.hip", "uneven_kernels.hip", "noalloc.hip", "serve_kernels.hip",
-               "large_transform_kernels.hip", "dit16_kernels.hip", "fixed_n512.hip",
+               "large_transform_kernels.hip", "dit16_kernels.hip", "fixed_n256.hip", "fixed_n512.hip",
                "fixed_n1024.hip", "fixed_n2048.hip", "fixed_n4096.hip",
                "fixed_n8192.hip"]
 HOST_HIP_SOURCES = ["api.cpp", "fft128_api.cpp", "products_api.cpp", "uneven_api.cpp",

@@ -185,6 +185,7 @@ namespace {
 bool fixed_fft(const Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
                const dc64 *tw, hipStream_t s, hipError_t *e) {
   switch (prog.log2n) {
+    case 8: return fixed_fft_n256(prog, data, ntrans, stride, tw, s, e);
     case 9: return fixed_fft_n512(prog, data, ntrans, stride, tw, s, e);
     case 10: return fixed_fft_n1024(prog, data, ntrans, stride, tw, s, e);
     case 11: return fixed_fft_n2048(prog, data, ntrans, stride, tw, s, e);
@@ -198,6 +199,7 @@ bool fixed_polymul(const Program &fwd, const Program &inv, const uint64_t *a, co
                    uint64_t *c, uint64_t batch, const dc64 *tw, const dc64 *twist,
                    const dc64 *untwist, hipStream_t s, hipError_t *e) {
   switch (fwd.log2n) {
+    case 8: return fixed_polymul_n256(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     case 9: return fixed_polymul_n512(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     case 10: return fixed_polymul_n1024(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);
     case 11: return fixed_polymul_n2048(fwd, inv, a, b, c, batch, tw, twist, untwist, s, e);

@@ -5,7 +5,7 @@
 // kernels.  With every pass
 // field a constant the passes unroll, positions and twiddle indices fold and
 // no pass descriptor is read at run time; the operations, and so the bits,
-// are the runtime executor's.  fixed_n512.hip ... fixed_n8192.hip instantiate
+// are the runtime executor's.  fixed_n256.hip ... fixed_n8192.hip instantiate
 // them for the plans listed there.
 #pragma once
 

@@ -70,6 +70,12 @@ bool fixed_polymul_n8192(const cfft_prog::Program &fwd, const cfft_prog::Program
                          hipError_t *e);
 bool fixed_fft_n8192(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,
                      uint64_t stride, const dc64 *tw, hipStream_t s, hipError_t *e);
+bool fixed_polymul_n256(const cfft_prog::Program &fwd, const cfft_prog::Program &inv,
+                        const uint64_t *a, const int64_t *b, uint64_t *c, uint64_t batch,
+                        const dc64 *tw, const dc64 *twist, const dc64 *untwist, hipStream_t s,
+                        hipError_t *e);
+bool fixed_fft_n256(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
+                    const dc64 *tw, hipStream_t s, hipError_t *e);
 bool fixed_fft_n512(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans, uint64_t stride,
                     const dc64 *tw, hipStream_t s, hipError_t *e);
 bool fixed_fft_n1024(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans,

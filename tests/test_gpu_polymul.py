@@ -32,7 +32,8 @@ def torus_err(c, exact):
                                            (4096, 7, 1024), (4096, 2, 512), (4096, 4, 512),
                                            (4096, 6, 512), (4096, 7, 512), (8192, 5, 1024),
                                            (8192, 2, 512), (8192, 7, 512), (16384, 5, 1024),
-                                           (16384, 0, 512), (16384, 6, 512)])
+                                           (16384, 0, 512), (16384, 6, 512), (512, 0, 256),
+                                           (512, 3, 256), (512, 5, 256), (512, 7, 256)])
 def test_polymul_bit_exact_vs_oracle(cfft, oracle, N, algo, base_n):
     from cfft_amd import products
     from cfft_amd.unordered import FftAlgo, Method, Plan
