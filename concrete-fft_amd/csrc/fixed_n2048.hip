@@ -1,5 +1,6 @@
 // fixed_n2048.hip -- compile-time pass programs (fixed_exec.hpp) for n = 2048:
-// every algo on bases 1024 and 512 except Dif16 / 1024 (register-resident).
+// every algo on bases 1024 and 512 (and, for the transforms, the ordered
+// plans' base_n = n) except Dif16 / 1024 (register-resident).
 #include "fixed_exec.hpp"
 
 namespace cfft_launch {
@@ -46,7 +47,15 @@ bool fixed_fft_n2048(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans
          fixed_fft_try<4, 11, 9>(prog, d, ntrans, stride, t, s, e) ||
          fixed_fft_try<5, 11, 9>(prog, d, ntrans, stride, t, s, e) ||
          fixed_fft_try<6, 11, 9>(prog, d, ntrans, stride, t, s, e) ||
-         fixed_fft_try<7, 11, 9>(prog, d, ntrans, stride, t, s, e);
+         fixed_fft_try<7, 11, 9>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<0, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<1, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<2, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<3, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<4, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<5, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<6, 11, 11>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<7, 11, 11>(prog, d, ntrans, stride, t, s, e);
 }
 
 }  // namespace cfft_launch

@@ -1,5 +1,6 @@
 // fixed_n4096.hip -- compile-time pass programs (fixed_exec.hpp) for n = 4096:
-// every algo on bases 1024 and 512 except Dif16 / 1024 (register-resident).
+// every algo on bases 1024 and 512 (and, for the transforms, the ordered
+// plans' base_n = n) except Dif16 / 1024 (register-resident).
 #include "fixed_exec.hpp"
 
 namespace cfft_launch {
@@ -46,7 +47,15 @@ bool fixed_fft_n4096(const cfft_prog::Program &prog, dc64 *data, uint64_t ntrans
          fixed_fft_try<4, 12, 9>(prog, d, ntrans, stride, t, s, e) ||
          fixed_fft_try<5, 12, 9>(prog, d, ntrans, stride, t, s, e) ||
          fixed_fft_try<6, 12, 9>(prog, d, ntrans, stride, t, s, e) ||
-         fixed_fft_try<7, 12, 9>(prog, d, ntrans, stride, t, s, e);
+         fixed_fft_try<7, 12, 9>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<0, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<1, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<2, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<3, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<4, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<5, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<6, 12, 12>(prog, d, ntrans, stride, t, s, e) ||
+         fixed_fft_try<7, 12, 12>(prog, d, ntrans, stride, t, s, e);
 }
 
 }  // namespace cfft_launch
